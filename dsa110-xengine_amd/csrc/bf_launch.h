@@ -1,0 +1,65 @@
+/*
+ * bf_launch.h -- internal C ABI between the C host layer (host/bf_plan.c)
+ * and the HIP kernels (csrc/bf_kernels.hip).  Not part of the public API.
+ */
+#ifndef BF_LAUNCH_H
+#define BF_LAUNCH_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Derived dimensions shared by every kernel. */
+typedef struct bfk_dims {
+  int nant;       /* antennas in the block                                */
+  int nant_pad;   /* antennas rounded up to 64 (MFMA K chunk)             */
+  int nkc;        /* nant_pad / 64                                        */
+  int nchan;      /* fine channels                                        */
+  int ngroup;     /* nchan / 8                                            */
+  int npkt;       /* packet-times (2 samples each)                        */
+  int nsamp;      /* 2 * npkt                                             */
+  int nbeam;      /* beams                                                */
+  int nbeam_pad;  /* beams rounded up to the beam set (256)               */
+  int nbt;        /* nbeam_pad / 32 beam tiles                            */
+  int tavg;       /* samples per output                                   */
+  int nout;       /* nsamp / tavg                                         */
+  int ncal;       /* weight channels: ngroup (group mode) or nchan (fine) */
+  int wmode;      /* 0 group, 1 fine                                      */
+  int amode;      /* 0 theta, 1 sin(theta)                                */
+  int prec;       /* 0 FP16, 1 FP8 hi+lo                                  */
+  int order;      /* 0 capture, 1 split -r                                */
+  int stuff;      /* 0 none, 1 antennas, 2 incoherent beam 0              */
+} bfk_dims;
+
+/* Bytes of the weight fragment buffer and of the per-(channel,pol,beam)
+ * weight scale buffer for dims. */
+size_t bfk_wfrag_bytes(const bfk_dims *d);
+size_t bfk_wscale_bytes(const bfk_dims *d);
+
+/* Beam weights from geometry, FP64 phase (device pointers). */
+int bfk_weights(const bfk_dims *d, const float *d_antpos, const float *d_cal,
+                const double *d_freqs, const double *d_theta, void *d_wfrag,
+                float *d_wscale, void *stream);
+
+/* Fused unpack -> MFMA beam GEMM -> |B|^2 -> average -> requantise. */
+int bfk_beamform(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
+                 const float *d_wscale, const float *d_scale, uint8_t *d_out,
+                 float *d_power, void *stream);
+
+/* Antenna-stuffing (-s) and incoherent-beam (-q) outputs. */
+int bfk_stuff(const bfk_dims *d, const uint8_t *d_in, const float *d_scale,
+              uint8_t *d_out, float *d_power, void *stream);
+
+/* beam_power[b] = sum over outputs and groups of power (deterministic). */
+int bfk_beam_power(const bfk_dims *d, const float *d_power, float *d_beam_power,
+                   void *stream);
+
+/* Name of the last HIP error seen by a launcher (thread-local). */
+const char *bfk_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

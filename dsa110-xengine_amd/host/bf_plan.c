@@ -1,0 +1,234 @@
+/*
+ * bf_plan.c -- the plan: device buffers, weights upload and the per-block
+ * sequence of kernel launches (C, HIP runtime C API).
+ *
+ * Replaces the reference's allocation block (dsaX_beamformer.cu:880-925),
+ * weight refresh (:948-951) and per-int launch loop (:955-1003).  One block
+ * is one fused beamformer launch (plus the -s/-q kernel when asked and a
+ * per-beam power reduction), instead of 5 launches per 32-sample int.
+ */
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../csrc/bf_launch.h"
+#include "bf_internal.h"
+#include "dsaxbf.h"
+
+struct dsaxbf_plan {
+  dsaxbf_config cfg;
+  bfk_dims dims;
+  hipStream_t stream;
+  void *d_wfrag;
+  float *d_wscale;
+  uint8_t *d_in;        /* staging for dsaxbf_process_block */
+  uint8_t *d_out;
+  float *d_power;       /* internal power buffer (always available) */
+  float *d_beam_power;
+  float *d_scale;
+  int have_weights;
+};
+
+#define HIPCHK(call, what)                                                      \
+  do {                                                                          \
+    hipError_t e_ = (call);                                                     \
+    if (e_ != hipSuccess)                                                       \
+      return dsaxbf_fail(DSAXBF_EHIP, "%s: %s", what, hipGetErrorString(e_));   \
+  } while (0)
+
+static void fill_dims(const dsaxbf_config *c, bfk_dims *d) {
+  memset(d, 0, sizeof(*d));
+  d->nant = c->nant;
+  d->nant_pad = (c->nant + 63) / 64 * 64;
+  d->nkc = d->nant_pad / 64;
+  d->nchan = c->nchan;
+  d->ngroup = c->nchan / 8;
+  d->npkt = c->npkt;
+  d->nsamp = 2 * c->npkt;
+  d->nbeam = c->nbeam;
+  d->nbeam_pad = (c->nbeam + 255) / 256 * 256;
+  d->nbt = d->nbeam_pad / 32;
+  d->tavg = c->tavg;
+  d->nout = d->nsamp / c->tavg;
+  d->ncal = c->weight_mode == DSAXBF_WEIGHTS_GROUP ? d->ngroup : c->nchan;
+  d->wmode = c->weight_mode;
+  d->amode = c->angle_mode;
+  d->prec = c->precision;
+  d->order = c->input_order;
+  d->stuff = c->stuffants;
+}
+
+void dsaxbf_destroy(dsaxbf_plan *p) {
+  if (!p) return;
+  hipSetDevice(p->cfg.device);
+  if (p->stream) hipStreamSynchronize(p->stream);
+  hipFree(p->d_wfrag);
+  hipFree(p->d_wscale);
+  hipFree(p->d_in);
+  hipFree(p->d_out);
+  hipFree(p->d_power);
+  hipFree(p->d_beam_power);
+  hipFree(p->d_scale);
+  if (p->stream) hipStreamDestroy(p->stream);
+  free(p);
+}
+
+int dsaxbf_create(dsaxbf_plan **out, const dsaxbf_config *cfg) {
+  dsaxbf_clear_error();
+  if (!out) return dsaxbf_fail(DSAXBF_EINVAL, "plan pointer is NULL");
+  *out = NULL;
+  int rc = dsaxbf_check_config(cfg);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0)
+    return dsaxbf_fail(DSAXBF_ENODEV, "no HIP device %d (found %d)", cfg->device, ndev);
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, cfg->device), "hipGetDeviceProperties");
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return dsaxbf_fail(DSAXBF_ENODEV, "device %d is %s; kernels are built for gfx950", cfg->device,
+                       prop.gcnArchName);
+  HIPCHK(hipSetDevice(cfg->device), "hipSetDevice");
+  dsaxbf_plan *p = calloc(1, sizeof(*p));
+  if (!p) return dsaxbf_fail(DSAXBF_ENOMEM, "plan allocation failed");
+  p->cfg = *cfg;
+  fill_dims(cfg, &p->dims);
+  const bfk_dims *d = &p->dims;
+  size_t nin = dsaxbf_input_bytes(cfg), nout = dsaxbf_output_bytes(cfg);
+  hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&p->d_wfrag, bfk_wfrag_bytes(d));
+  if (e == hipSuccess) e = hipMalloc((void **)&p->d_wscale, bfk_wscale_bytes(d));
+  if (e == hipSuccess) e = hipMalloc((void **)&p->d_in, nin);
+  if (e == hipSuccess) e = hipMalloc((void **)&p->d_out, nout);
+  if (e == hipSuccess) e = hipMalloc((void **)&p->d_power, nout * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void **)&p->d_beam_power, (size_t)cfg->nbeam * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void **)&p->d_scale, (size_t)cfg->nbeam * sizeof(float));
+  if (e != hipSuccess) {
+    dsaxbf_destroy(p);
+    return dsaxbf_fail(DSAXBF_ENOMEM, "device allocation failed: %s", hipGetErrorString(e));
+  }
+  *out = p;
+  return DSAXBF_OK;
+}
+
+int dsaxbf_get_config(const dsaxbf_plan *p, dsaxbf_config *cfg) {
+  if (!p || !cfg) return dsaxbf_fail(DSAXBF_EINVAL, "NULL argument");
+  *cfg = p->cfg;
+  return DSAXBF_OK;
+}
+
+void *dsaxbf_get_stream(dsaxbf_plan *p) { return p ? (void *)p->stream : NULL; }
+
+int dsaxbf_set_weights(dsaxbf_plan *p, const float *antpos, const float *cal,
+                       const double *freqs_hz, const double *theta_rad, double sep_arcmin) {
+  dsaxbf_clear_error();
+  if (!p || !antpos || !cal || !freqs_hz) return dsaxbf_fail(DSAXBF_EINVAL, "NULL argument");
+  const bfk_dims *d = &p->dims;
+  HIPCHK(hipSetDevice(p->cfg.device), "hipSetDevice");
+  size_t ncal_f = (size_t)d->nant * d->ncal * 4;
+  double *theta = malloc(sizeof(double) * d->nbeam);
+  if (!theta) return dsaxbf_fail(DSAXBF_ENOMEM, "host allocation failed");
+  for (int b = 0; b < d->nbeam; b++)
+    theta[b] = theta_rad ? theta_rad[b] : sep_arcmin * (double)(d->nbeam / 2 - 1 - b) * M_PI / 10800.0;
+  float *d_antpos = NULL, *d_cal = NULL;
+  double *d_freqs = NULL, *d_theta = NULL;
+  hipError_t e = hipMalloc((void **)&d_antpos, sizeof(float) * d->nant);
+  if (e == hipSuccess) e = hipMalloc((void **)&d_cal, sizeof(float) * ncal_f);
+  if (e == hipSuccess) e = hipMalloc((void **)&d_freqs, sizeof(double) * d->nchan);
+  if (e == hipSuccess) e = hipMalloc((void **)&d_theta, sizeof(double) * d->nbeam);
+  if (e == hipSuccess) e = hipMemcpy(d_antpos, antpos, sizeof(float) * d->nant, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_cal, cal, sizeof(float) * ncal_f, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_freqs, freqs_hz, sizeof(double) * d->nchan, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_theta, theta, sizeof(double) * d->nbeam, hipMemcpyHostToDevice);
+  free(theta);
+  int rc = DSAXBF_OK;
+  if (e != hipSuccess) {
+    rc = dsaxbf_fail(DSAXBF_EHIP, "weights upload: %s", hipGetErrorString(e));
+  } else {
+    rc = bfk_weights(d, d_antpos, d_cal, d_freqs, d_theta, p->d_wfrag, p->d_wscale, p->stream);
+    if (rc) rc = dsaxbf_fail(rc, "%s", bfk_last_error());
+    else {
+      e = hipStreamSynchronize(p->stream);
+      if (e != hipSuccess) rc = dsaxbf_fail(DSAXBF_EHIP, "weights kernel: %s", hipGetErrorString(e));
+    }
+  }
+  hipFree(d_antpos);
+  hipFree(d_cal);
+  hipFree(d_freqs);
+  hipFree(d_theta);
+  if (rc == DSAXBF_OK) p->have_weights = 1;
+  return rc;
+}
+
+int dsaxbf_execute(dsaxbf_plan *p, const uint8_t *d_in, const float *d_scale, uint8_t *d_out,
+                   float *d_power, float *d_beam_power, void *stream) {
+  if (!p || !d_in || !d_scale || !d_out) return dsaxbf_fail(DSAXBF_EINVAL, "NULL argument");
+  const bfk_dims *d = &p->dims;
+  if (!p->have_weights && d->stuff != DSAXBF_STUFF_ANTS)
+    return dsaxbf_fail(DSAXBF_ESTATE, "execute before dsaxbf_set_weights");
+  void *s = stream ? stream : (void *)p->stream;
+  float *pw = d_power ? d_power : p->d_power;
+  int rc;
+  if (d->stuff != DSAXBF_STUFF_ANTS) {
+    rc = bfk_beamform(d, d_in, p->d_wfrag, p->d_wscale, d_scale, d_out, pw, s);
+    if (rc) return dsaxbf_fail(rc, "%s", bfk_last_error());
+  }
+  if (d->stuff != DSAXBF_STUFF_NONE) {
+    rc = bfk_stuff(d, d_in, d_scale, d_out, pw, s);
+    if (rc) return dsaxbf_fail(rc, "%s", bfk_last_error());
+  }
+  if (d_beam_power) {
+    rc = bfk_beam_power(d, pw, d_beam_power, s);
+    if (rc) return dsaxbf_fail(rc, "%s", bfk_last_error());
+  }
+  return DSAXBF_OK;
+}
+
+int dsaxbf_process_block(dsaxbf_plan *p, const uint8_t *h_in, const float *h_scale,
+                         uint8_t *h_out, float *h_power, float *h_beam_power) {
+  dsaxbf_clear_error();
+  if (!p || !h_in || !h_scale || !h_out) return dsaxbf_fail(DSAXBF_EINVAL, "NULL argument");
+  HIPCHK(hipSetDevice(p->cfg.device), "hipSetDevice");
+  size_t nin = dsaxbf_input_bytes(&p->cfg), nout = dsaxbf_output_bytes(&p->cfg);
+  hipStream_t s = p->stream;
+  HIPCHK(hipMemcpyAsync(p->d_in, h_in, nin, hipMemcpyHostToDevice, s), "H2D block");
+  HIPCHK(hipMemcpyAsync(p->d_scale, h_scale, sizeof(float) * p->cfg.nbeam, hipMemcpyHostToDevice, s),
+         "H2D scale");
+  int rc = dsaxbf_execute(p, p->d_in, p->d_scale, p->d_out, p->d_power,
+                          h_beam_power ? p->d_beam_power : NULL, s);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(h_out, p->d_out, nout, hipMemcpyDeviceToHost, s), "D2H output");
+  if (h_power)
+    HIPCHK(hipMemcpyAsync(h_power, p->d_power, nout * sizeof(float), hipMemcpyDeviceToHost, s), "D2H power");
+  if (h_beam_power)
+    HIPCHK(hipMemcpyAsync(h_beam_power, p->d_beam_power, sizeof(float) * p->cfg.nbeam,
+                          hipMemcpyDeviceToHost, s), "D2H beam power");
+  HIPCHK(hipStreamSynchronize(s), "block");
+  return DSAXBF_OK;
+}
+
+/* ---- device memory helpers ------------------------------------------------ */
+int dsaxbf_dev_alloc(void **d_ptr, size_t bytes, int device) {
+  if (!d_ptr) return dsaxbf_fail(DSAXBF_EINVAL, "NULL argument");
+  HIPCHK(hipSetDevice(device), "hipSetDevice");
+  HIPCHK(hipMalloc(d_ptr, bytes), "hipMalloc");
+  return DSAXBF_OK;
+}
+int dsaxbf_dev_free(void *d_ptr) {
+  HIPCHK(hipFree(d_ptr), "hipFree");
+  return DSAXBF_OK;
+}
+int dsaxbf_copy_h2d(void *d_dst, const void *h_src, size_t bytes, void *stream) {
+  HIPCHK(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream), "H2D");
+  HIPCHK(hipStreamSynchronize((hipStream_t)stream), "H2D sync");
+  return DSAXBF_OK;
+}
+int dsaxbf_copy_d2h(void *h_dst, const void *d_src, size_t bytes, void *stream) {
+  HIPCHK(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream), "D2H");
+  HIPCHK(hipStreamSynchronize((hipStream_t)stream), "D2H sync");
+  return DSAXBF_OK;
+}
+int dsaxbf_sync(void *stream) {
+  HIPCHK(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+  return DSAXBF_OK;
+}

@@ -1,0 +1,249 @@
+/*
+ * dsaxbf.h -- C ABI of the MI355X (gfx950) DSA X-engine coherent beamformer.
+ *
+ * The reference beamformer is a single process, src/dsaX_beamformer.cu, that
+ * reads psrdada ring REORDER_BLOCK_KEY2 (0xbada) and writes BF_BLOCK_KEY
+ * (0xdcda).  It has no library interface of its own; every entry point below
+ * replaces one step of its per-block loop (dsaX_beamformer.cu:939-1090) and is
+ * what a host program (the dsaX_beamformer-compatible CLI in this package, or a
+ * cgo/ctypes caller) binds.  Each declaration cites the reference code it
+ * stands in for.
+ *
+ * Conventions
+ *   - Return value 0 is success; negative values are DSAXBF_E* codes.  The
+ *     library never prints and never exits; dsaxbf_last_error() gives a
+ *     message for the last failure on the calling thread.
+ *   - A plan owns its device buffers and weights.  Callers own host blocks.
+ *   - A plan is not thread-safe: use one plan per host thread.
+ *   - Pointers named d_* are device pointers (HBM); h_* are host pointers.
+ *   - "stream" is a hipStream_t passed as void* (NULL = the plan's stream).
+ *
+ * Data layout (all byte orders are those of the reference pipeline)
+ *   input  : capture order [npkt][nant][nchan][2 times][2 pols], one byte per
+ *            complex sample, low nibble = real, high nibble = imaginary, both
+ *            4-bit two's complement (dsaX_beamformer.cu:182-191,
+ *            utils/gen_packet.py:196-206).  With input_order =
+ *            DSAXBF_ORDER_SPLIT_R the block is [npkt][nchan/8][nant][8][2][2]
+ *            (what dsaX_split -r writes, dsaX_split.c:86-101).
+ *   output : [nsamp/tavg][nbeam][nchan/8] uint8 (BF_BLOCK_KEY layout,
+ *            dsaX_beamformer.cu:150-156,986-996), nsamp = 2*npkt.
+ *   power  : same index order as output, float32, the averaged |B|^2 before
+ *            scaling (the reference's adder input sum, dsaX_beamformer.cu:139-147).
+ */
+#ifndef DSAXBF_H
+#define DSAXBF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---------------------------------------------------- */
+#define DSAXBF_OK        0
+#define DSAXBF_EINVAL   -1   /* bad argument or unsupported shape            */
+#define DSAXBF_ENOMEM   -2   /* host or device allocation failed              */
+#define DSAXBF_EHIP     -3   /* a HIP runtime call failed                     */
+#define DSAXBF_EIO      -4   /* file missing, short or malformed              */
+#define DSAXBF_ESTATE   -5   /* call out of order (e.g. execute before weights)*/
+#define DSAXBF_ENODEV   -6   /* no usable gfx950 device / kernels not loaded  */
+
+/* ---- reference constants (dsaX_def.h:16,22,28-29,72,80-84) ---------- */
+#define DSAXBF_REORDER_BLOCK_KEY2 0x0000bada
+#define DSAXBF_BF_BLOCK_KEY       0x0000dcda
+#define DSAXBF_CVAC               299792458.0
+#define DSAXBF_NPACKETS           2048   /* packet-times per block           */
+#define DSAXBF_NW                 48     /* output channels per 384          */
+#define DSAXBF_NANT_FILE          64     /* antennas in the weights file     */
+#define DSAXBF_NBP                8      /* AGC history length               */
+#define DSAXBF_FAVG               8      /* fine channels per output channel */
+
+/* ---- enumerations ---------------------------------------------------- */
+enum dsaxbf_precision {
+  DSAXBF_PREC_FP16    = 0,  /* FP16 weights x FP16 voltages, FP32 accumulate   */
+  DSAXBF_PREC_FP8SPLIT = 1  /* FP8 E4M3 voltages x (hi+lo) FP8 weights, FP32   */
+};
+enum dsaxbf_weight_mode {
+  DSAXBF_WEIGHTS_GROUP = 0, /* one weight per 8-channel group at freqs[8g+4]
+                               (dsaX_beamformer.cu:441)                       */
+  DSAXBF_WEIGHTS_FINE  = 1  /* one weight per fine channel                    */
+};
+enum dsaxbf_angle_mode {
+  DSAXBF_ANGLE_THETA = 0,   /* phase uses theta itself (dsaX_beamformer.cu:440-441) */
+  DSAXBF_ANGLE_SIN   = 1    /* phase uses sin(theta) (dsaX_beamformer_offline.cu:422) */
+};
+enum dsaxbf_stuff {
+  DSAXBF_STUFF_NONE = 0,    /* normal beams                                   */
+  DSAXBF_STUFF_ANTS = 1,    /* -s: beam b carries one antenna's power
+                               (dsaX_beamformer.cu:326-337,400-404)           */
+  DSAXBF_STUFF_IB   = 2     /* -q: beam 0 carries the incoherent sum
+                               (dsaX_beamformer.cu:307-323,405-414)           */
+};
+enum dsaxbf_input_order {
+  DSAXBF_ORDER_CAPTURE = 0, /* [pkt][ant][chan][t][pol] (dsaX_split without -r) */
+  DSAXBF_ORDER_SPLIT_R = 1  /* [pkt][chan/8][ant][8][t][pol] (dsaX_split -r)    */
+};
+
+/* ---- configuration --------------------------------------------------- */
+typedef struct dsaxbf_config {
+  int nant;         /* antennas in the input block, 1..256                   */
+  int nchan;        /* fine channels in the block, multiple of 8             */
+  int npkt;         /* packet-times per block (2 samples each), mult. of 32  */
+  int nbeam;        /* beams formed, 1..1024                                  */
+  int tavg;         /* samples averaged per output, one of 8,16,32,64        */
+  int precision;    /* enum dsaxbf_precision                                 */
+  int weight_mode;  /* enum dsaxbf_weight_mode                               */
+  int angle_mode;   /* enum dsaxbf_angle_mode                                */
+  int stuffants;    /* enum dsaxbf_stuff                                     */
+  int input_order;  /* enum dsaxbf_input_order                               */
+  int device;       /* HIP device ordinal                                     */
+} dsaxbf_config;
+
+typedef struct dsaxbf_plan dsaxbf_plan;
+
+/* Fill cfg with the reference's own compiled-in shape: 2048 packets, 384
+ * channels, 256 beams, 8-sample averaging, group weights, theta angles,
+ * capture order, FP16 (dsaX_beamformer.cu:139-156,420-449; dsaX_def.h:72,80). */
+void dsaxbf_config_default(dsaxbf_config *cfg, int nant);
+
+/* Bytes of one input block and one output block for cfg. */
+size_t dsaxbf_input_bytes(const dsaxbf_config *cfg);
+size_t dsaxbf_output_bytes(const dsaxbf_config *cfg);
+
+/* Message describing the most recent failure on this thread ("" if none). */
+const char *dsaxbf_last_error(void);
+
+/* ---- plan lifecycle (replaces dsaX_beamformer.cu:880-925 allocations) --- */
+int  dsaxbf_create(dsaxbf_plan **plan, const dsaxbf_config *cfg);
+void dsaxbf_destroy(dsaxbf_plan *plan);     /* dsaX_beamformer.cu:1092-1122 */
+int  dsaxbf_get_config(const dsaxbf_plan *plan, dsaxbf_config *cfg);
+void *dsaxbf_get_stream(dsaxbf_plan *plan); /* the plan's hipStream_t        */
+
+/* ---- weights (replaces init_weights + calc_weights, dsaX_beamformer.cu:
+ *      420-449,559-602,948-951) ------------------------------------------ */
+
+/* Read the reference weights file: 64 float antenna eastings (m) followed by
+ * cal[64 ant][48 group][2 pol][re,im] float32, each complex normalised as
+ * w/|w|^2 (zero stays zero), then zero every antenna listed in flagfile (one
+ * integer per line; NULL or "nofile" = none).  Unlike the reference, a missing
+ * or short file, or a flag index outside 0..63, is an error (DSAXBF_EIO /
+ * DSAXBF_EINVAL) instead of silently leaving garbage.
+ * antpos: 64 floats out; cal: 64*48*2*2 floats out. */
+int dsaxbf_read_weights_file(const char *fnam, const char *flagfile,
+                             float *antpos, float *cal);
+
+/* Compute the beam weights on the device: W[b][a] = exp(i*phi)*cal[a], with
+ * phi = -2*pi*f*ang_b*x_a/c evaluated in FP64 (ang_b = theta_b or
+ * sin(theta_b) per angle_mode), then converted to the plan's MFMA operand
+ * format (FP16, or FP8 E4M3 hi+lo with a power-of-two scale per beam).
+ *   antpos   : [nant] eastings in metres
+ *   cal      : [nant][ncal][2 pol][re,im], ncal = nchan/8 (group mode) or
+ *              nchan (fine mode); already normalised, flagged antennas zero
+ *   freqs_hz : [nchan] sky frequency of each fine channel (Hz)
+ *   theta_rad: [nbeam] beam angles (radians); NULL selects the reference
+ *              fan theta_b = sep_arcmin*(nbeam/2-1-b)*pi/10800
+ *              (dsaX_beamformer.cu:440, sep = 1 arcmin there).
+ * Host pointers.  Synchronous. */
+int dsaxbf_set_weights(dsaxbf_plan *plan, const float *antpos, const float *cal,
+                       const double *freqs_hz, const double *theta_rad,
+                       double sep_arcmin);
+
+/* Reference channel frequencies: freqs[i] = (fch1_mhz - i*250/8192)*1e6
+ * (dsaX_beamformer.cu:799-800). */
+void dsaxbf_reference_freqs(double fch1_mhz, int nchan, double *freqs_hz);
+
+/* ---- one block (replaces the 128-int loop, dsaX_beamformer.cu:955-1003,
+ *      i.e. promoter + beamformer + badder + adder) ---------------------- */
+
+/* Beamform one block already resident in device memory.
+ *   d_in        : input block (dsaxbf_input_bytes bytes)
+ *   d_scale     : [nbeam] float per-beam requantisation scale (the AGC
+ *                 output; d_bp in the reference)
+ *   d_out       : output block, uint8 (dsaxbf_output_bytes bytes)
+ *   d_power     : optional float power before scaling, same shape as d_out
+ *   d_beam_power: optional [nbeam] float, sum of power over the block (the
+ *                 AGC input; d_added + calc_allbp in the reference)
+ * Asynchronous on stream. */
+int dsaxbf_execute(dsaxbf_plan *plan, const uint8_t *d_in, const float *d_scale,
+                   uint8_t *d_out, float *d_power, float *d_beam_power,
+                   void *stream);
+
+/* Same as dsaxbf_execute with host pointers: copies the block in, runs it,
+ * copies the output (and optional power / beam_power) back, then waits.
+ * h_scale is [nbeam] floats on the host.  This is the psrdada-facing call
+ * (ipcio_open_block_read pointer in, ipcio_write buffer out). */
+int dsaxbf_process_block(dsaxbf_plan *plan, const uint8_t *h_in,
+                         const float *h_scale, uint8_t *h_out,
+                         float *h_power, float *h_beam_power);
+
+/* ---- AGC (dsaX_beamformer.cu:482-530,1005-1066) ----------------------- */
+typedef struct dsaxbf_agc {
+  int    nbeam;
+  int    ngroup;          /* output channels (48 in the reference)           */
+  int    nints;           /* 32-sample integrations per block (128)          */
+  int    enabled;         /* 0 = -g: hold the first block's median scale     */
+  int    started;         /* 0 before the first block, then 1, then 2        */
+  int    bpctr;           /* blocks seen                                      */
+  float  medbp;           /* median of the first block's beam powers         */
+  float *many_bp;         /* [DSAXBF_NBP][nbeam] history ring                */
+  float *bp;              /* [nbeam] work                                     */
+} dsaxbf_agc;
+
+int  dsaxbf_agc_init(dsaxbf_agc *agc, int nbeam, int ngroup, int nints, int enabled);
+void dsaxbf_agc_free(dsaxbf_agc *agc);
+/* Feed one block's raw per-beam power sum; writes the scale to use for the
+ * NEXT block into scale_out[nbeam].  *zero_output is set to 1 when the block
+ * just processed must be written as zeros (the first block, :1021-1033). */
+int  dsaxbf_agc_update(dsaxbf_agc *agc, const float *beam_power_raw,
+                       float *scale_out, int *zero_output);
+
+/* ---- requantisation of one value (dsaX_beamformer.cu:150-156) ---------- */
+/* (uint8)(rint(x*scale) / 2) with C integer division and wrap modulo 256. */
+uint8_t dsaxbf_requant(float x, float scale);
+
+/* ---- synthetic inputs (one seeded generator for every harness) --------- */
+/* Fill a block with 4-bit complex Gaussian samples (sigma per component),
+ * rounded and clipped to [-clip, clip], in capture order; seeded
+ * (splitmix64 -> xoshiro256**, Box-Muller in FP64).  If tone_snr > 0 a
+ * narrowband tone from direction tone_theta (radians, sin geometry) is added
+ * before quantisation to channels tone_chan[0..ntone-1], with per-antenna
+ * delays from antpos and freqs, divided by the complex antenna gain
+ * gain[nant][re,im] (NULL = 1) so that weights built with cal = gain steer
+ * the tone coherently.  SNR is tone power over noise power per sample.
+ * Multithreaded over packets; the result does not depend on thread count. */
+int dsaxbf_synth_block(const dsaxbf_config *cfg, uint64_t seed, double sigma,
+                       int clip, double tone_snr, double tone_theta,
+                       const int *tone_chan, int ntone, const float *antpos,
+                       const float *gain, const double *freqs_hz,
+                       uint8_t *h_block);
+
+/* Geometry of the benchmark workloads: antenna eastings uniform on
+ * [0, length_m] (seeded), unit-amplitude-ish gains g_a = (1 +- amp_jitter)
+ * * exp(i*U[0,2pi)) written as cal[nant][ncal][2][2] (same gain for every
+ * channel and pol), and beam angles uniform in +-half_span_deg. */
+int dsaxbf_synth_geometry(int nant, int ncal, int nbeam, uint64_t seed,
+                          double length_m, double amp_jitter,
+                          double half_span_deg, float *antpos, float *cal,
+                          double *theta_rad);
+
+/* ---- device memory helpers (for bindings without a GPU array library) --- */
+int dsaxbf_dev_alloc(void **d_ptr, size_t bytes, int device);
+int dsaxbf_dev_free(void *d_ptr);
+int dsaxbf_copy_h2d(void *d_dst, const void *h_src, size_t bytes, void *stream);
+int dsaxbf_copy_d2h(void *h_dst, const void *d_src, size_t bytes, void *stream);
+int dsaxbf_sync(void *stream);
+
+/* ---- diagnostics (dsaX_beamformer.cu:217-238,971-972) ------------------ */
+/* ANTPOL_RMS of packet pkt, channel group g: rms[pol*64+ant] =
+ * sqrt(sum over 8 ch x 2 t of (re^2+im^2) / 32), host block pointer. */
+int dsaxbf_antpol_rms(const dsaxbf_config *cfg, const uint8_t *h_block,
+                      int pkt, int group, float *rms);
+
+/* Library build identification, e.g. "dsaxbf 1.0 gfx950". */
+const char *dsaxbf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSAXBF_H */

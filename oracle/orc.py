@@ -1,0 +1,150 @@
+"""ctypes wrapper of the FP64 oracle and the FP32 CPU baseline (oracle/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: the checker, never the product.  Imported only by
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+
+The C restatement is oracle/bf_oracle.c (FP64) and oracle/bf_cpu.c (FP32);
+each function there cites the reference file:line it follows.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liborc.so")
+
+
+class Dims(C.Structure):
+    _fields_ = [("nant", C.c_int), ("nchan", C.c_int), ("npkt", C.c_int), ("nbeam", C.c_int),
+                ("tavg", C.c_int), ("input_order", C.c_int)]
+
+
+class AgcState(C.Structure):
+    _fields_ = [("nbeam", C.c_int), ("ngroup", C.c_int), ("nints", C.c_int), ("enabled", C.c_int),
+                ("started", C.c_int), ("bpctr", C.c_int), ("medbp", C.c_float),
+                ("many_bp", C.c_float * (8 * 1024))]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"oracle not built: {LIB_PATH}")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.orc_unpack_byte.argtypes = [C.c_uint8, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_in_index.restype = C.c_long
+        L.orc_in_index.argtypes = [C.POINTER(Dims), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_weights.argtypes = [C.POINTER(Dims), C.c_int, C.c_int, P, P, P, P, P]
+        L.orc_ref_theta.argtypes = [C.c_int, C.c_double, P]
+        L.orc_power.argtypes = [C.POINTER(Dims), C.c_int, P, P, C.c_int, C.c_int, C.c_int, P]
+        L.orc_requant.restype = C.c_uint8
+        L.orc_requant.argtypes = [C.c_float, C.c_float]
+        L.orc_beam_power.argtypes = [C.POINTER(Dims), P, P]
+        L.orc_agc_init.argtypes = [C.POINTER(AgcState), C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_agc_update.restype = C.c_int
+        L.orc_agc_update.argtypes = [C.POINTER(AgcState), P, P]
+        L.cpu_beamform.restype = C.c_double
+        L.cpu_beamform.argtypes = [C.POINTER(Dims), C.c_int, P, P, P, C.c_int, C.c_int, P, P,
+                                   C.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def dims(nant, nchan, npkt, nbeam, tavg, input_order=0) -> Dims:
+    return Dims(nant, nchan, npkt, nbeam, tavg, input_order)
+
+
+def unpack_byte(v: int):
+    re, im = C.c_int(), C.c_int()
+    lib().orc_unpack_byte(v, C.byref(re), C.byref(im))
+    return re.value, im.value
+
+
+def in_index(d: Dims, p, a, c, t, q) -> int:
+    return int(lib().orc_in_index(C.byref(d), p, a, c, t, q))
+
+
+def ref_theta(nbeam: int, sep_arcmin: float = 1.0) -> np.ndarray:
+    t = np.empty(nbeam, np.float64)
+    lib().orc_ref_theta(nbeam, sep_arcmin, _p(t))
+    return t
+
+
+def weights(d: Dims, wmode: int, amode: int, antpos, cal, freqs, theta) -> np.ndarray:
+    """FP64 weights, shape [ncal][2][nbeam][nant] complex128."""
+    ncal = d.nchan // 8 if wmode == 0 else d.nchan
+    a = np.ascontiguousarray(antpos, np.float64)
+    c = np.ascontiguousarray(cal, np.float64)
+    f = np.ascontiguousarray(freqs, np.float64)
+    t = np.ascontiguousarray(theta, np.float64)
+    W = np.zeros((ncal, 2, d.nbeam, d.nant, 2), np.float64)
+    lib().orc_weights(C.byref(d), wmode, amode, _p(a), _p(c), _p(f), _p(t), _p(W))
+    return W[..., 0] + 1j * W[..., 1]
+
+
+def power(d: Dims, wmode: int, block, W, stuff=0, g0=0, g1=None) -> np.ndarray:
+    """FP64 averaged power [nout][nbeam][ngroup]; groups outside [g0,g1) are NaN."""
+    ng = d.nchan // 8
+    g1 = ng if g1 is None else g1
+    Wr = np.ascontiguousarray(np.stack([W.real, W.imag], -1), np.float64)
+    out = np.full((2 * d.npkt // d.tavg, d.nbeam, ng), np.nan, np.float64)
+    blk = np.ascontiguousarray(block, np.uint8)
+    lib().orc_power(C.byref(d), wmode, _p(blk), _p(Wr), stuff, g0, g1, _p(out))
+    return out
+
+
+def requant(x: float, scale: float) -> int:
+    return int(lib().orc_requant(x, scale))
+
+
+def requant_block(power, scale) -> np.ndarray:
+    """Vectorised restatement of orc_requant over [nout][nbeam][ngroup]."""
+    v = (power.astype(np.float32) * np.asarray(scale, np.float32)[None, :, None]).astype(np.float32)
+    r = np.rint(v.astype(np.float64))
+    r = np.where(np.isnan(r), 0, r)
+    r = np.clip(r, -2147483648.0, 2147483647.0).astype(np.int64)
+    h = np.trunc(r / 2).astype(np.int64)   # C division truncates toward zero
+    return (h & 0xFF).astype(np.uint8)
+
+
+def beam_power(d: Dims, pw) -> np.ndarray:
+    bp = np.zeros(d.nbeam, np.float64)
+    lib().orc_beam_power(C.byref(d), _p(np.ascontiguousarray(pw, np.float64)), _p(bp))
+    return bp
+
+
+class Agc:
+    def __init__(self, nbeam, ngroup, nints, enabled=True):
+        self.s = AgcState()
+        self.nbeam = nbeam
+        lib().orc_agc_init(C.byref(self.s), nbeam, ngroup, nints, int(enabled))
+
+    def update(self, bp_raw):
+        bp = np.ascontiguousarray(bp_raw, np.float32)
+        sc = np.empty(self.nbeam, np.float32)
+        z = lib().orc_agc_update(C.byref(self.s), _p(bp), _p(sc))
+        return sc, bool(z)
+
+
+def cpu_beamform(d: Dims, wmode: int, block, W, scale, g0=0, g1=None, nthreads=0):
+    """FP32 multithreaded CPU baseline; returns (seconds, power, out)."""
+    ng = d.nchan // 8
+    g1 = ng if g1 is None else g1
+    Wf = np.ascontiguousarray(np.stack([W.real, W.imag], -1), np.float32)
+    shape = (2 * d.npkt // d.tavg, d.nbeam, ng)
+    pw = np.zeros(shape, np.float32)
+    out = np.zeros(shape, np.uint8)
+    sc = np.ascontiguousarray(scale, np.float32)
+    secs = lib().cpu_beamform(C.byref(d), wmode, _p(np.ascontiguousarray(block, np.uint8)),
+                              _p(Wf), _p(sc), g0, g1, _p(pw), _p(out), nthreads)
+    return secs, pw, out
