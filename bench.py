@@ -1,0 +1,263 @@
+#!/usr/bin/env python3
+"""Beamformer benchmark: one step = one full DSA block beamformed on one GPU.
+
+Workload (synthetic, seeded): 64 antennas x 2 pols x 384 fine channels x 4096
+samples of 4-bit complex voltages (Gaussian sigma 2, clipped to +-7, a tone
+at SNR 10 in 4 channels from one beam direction), in capture packet order;
+256 beams over +-1.75 deg on a 1.2 km E-W line with one weight per fine
+channel; detect, sum 2 pols x 32 samples x 8 channels, requantise to uint8
+-> [128 t][256 beams][48 ch].  Input blocks are resident in HBM (three
+distinct blocks in rotation, larger than the 256 MB last-level cache) when
+the timed region starts.
+
+Both precisions run; "value" is the FP8-split path (FP8 E4M3 voltages x hi+lo
+FP8 weights, FP32 accumulate) unless --precision fp16.  The line also carries
+the FP16 path, the error of each against the FP64 oracle on a subset of
+channel groups, the MFMA roofline of the fused kernel (HIP events on the
+kernel's own stream) and a multithreaded FP32 CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "dsa110-xengine_amd", "python"), os.path.join(ROOT, "oracle")]
+
+METRIC = "beamformed input samples/s + % MFMA peak, FP8-split vs FP16, at +-1 LSB"
+PEAK_TFLOPS = {"fp16": 2500.0, "fp8split": 5000.0}   # MI355X dense MFMA
+NANT, NCHAN, NPKT, NBEAM, TAVG = 64, 384, 2048, 256, 32
+NSAMP = 2 * NPKT
+SAMPLES_PER_BLOCK = NANT * 2 * NCHAN * NSAMP
+FLOP_PER_BLOCK = 8.0 * NBEAM * NANT * NCHAN * 2 * NSAMP
+
+
+class HipEvents:
+    """hipEvent timing through the HIP runtime's C API."""
+
+    def __init__(self):
+        self.h = C.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.h.hipEventDestroy.argtypes = [C.c_void_p]
+        self.h.hipDeviceSynchronize.argtypes = []
+
+    def create(self, n):
+        evs = []
+        for _ in range(n):
+            e = C.c_void_p()
+            assert self.h.hipEventCreate(C.byref(e)) == 0
+            evs.append(e)
+        return evs
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, stream) == 0
+
+    def ms(self, a, b):
+        self.h.hipEventSynchronize(b)
+        x = C.c_float()
+        assert self.h.hipEventElapsedTime(C.byref(x), a, b) == 0
+        return x.value
+
+    def sync(self):
+        assert self.h.hipDeviceSynchronize() == 0
+
+
+def workload(dsaxbf, nblocks, seed=2024):
+    cfg = dsaxbf.config(nant=NANT, nchan=NCHAN, npkt=NPKT, nbeam=NBEAM, tavg=TAVG,
+                        weight_mode=dsaxbf.WEIGHTS_FINE, angle_mode=dsaxbf.ANGLE_SIN)
+    antpos, cal, theta = dsaxbf.synth_geometry(NANT, NCHAN, NBEAM, seed, 1200.0, 0.1, 1.75)
+    freqs = dsaxbf.reference_freqs(1498.75, NCHAN)
+    tone_beam = 77
+    gain = cal[:, 0, 0, :].copy()
+    blocks = [dsaxbf.synth_block(cfg, seed + 17 * k, 2.0, 7, 10.0, theta[tone_beam],
+                                 (40, 41, 200, 301), antpos, gain, freqs) for k in range(nblocks)]
+    return cfg, antpos, cal, theta, freqs, blocks, tone_beam
+
+
+def time_path(dsaxbf, ev, prec_name, cfg, antpos, cal, theta, freqs, blocks, steps, warmup):
+    cfg.precision = dsaxbf.PREC_FP16 if prec_name == "fp16" else dsaxbf.PREC_FP8SPLIT
+    plan = dsaxbf.Plan(cfg)
+    plan.set_weights(antpos, cal, freqs, theta)
+    stream = plan.stream
+    nout = cfg.nout * cfg.nbeam * cfg.ngroup
+    d_in = [dsaxbf.DeviceBuffer(b.nbytes) for b in blocks]
+    for buf, b in zip(d_in, blocks):
+        buf.upload(b)
+    d_scale = dsaxbf.DeviceBuffer(4 * NBEAM)
+    d_out = dsaxbf.DeviceBuffer(nout)
+    d_pow = dsaxbf.DeviceBuffer(4 * nout)
+    d_bp = dsaxbf.DeviceBuffer(4 * NBEAM)
+    # scale: the AGC's first-block rule (median beam power) on block 0
+    d_scale.upload(np.ones(NBEAM, np.float32))
+    plan.execute(d_in[0], d_scale, d_out, d_pow, d_bp, stream)
+    bp = d_bp.download(np.empty(NBEAM, np.float32), stream)
+    agc = dsaxbf.Agc(NBEAM, cfg.ngroup, NPKT // 16)
+    scale, _ = agc.update(bp)
+    d_scale.upload(scale)
+
+    def step(i, k0=None, k1=None):
+        # one block: fused beamformer (uint8 out, no float power) + AGC sums
+        if k0 is not None:
+            ev.record(k0, stream)
+        plan.execute(d_in[i % len(d_in)], d_scale, d_out, None, None, stream)
+        if k1 is not None:
+            ev.record(k1, stream)
+        plan.beam_power(d_bp, stream)
+
+    for i in range(warmup):
+        step(i)
+    ev.sync()
+    e0, e1 = ev.create(2)
+    ks = ev.create(2 * steps)
+    ev.record(e0, stream)
+    for i in range(steps):
+        step(i, ks[2 * i], ks[2 * i + 1])
+    ev.record(e1, stream)
+    ev.sync()
+    total_ms = ev.ms(e0, e1)
+    kern_ms = [ev.ms(ks[2 * i], ks[2 * i + 1]) for i in range(steps)]
+    # outputs of block 0 for the error report
+    plan.execute(d_in[0], d_scale, d_out, d_pow, None, stream)
+    out = d_out.download(np.empty((cfg.nout, NBEAM, cfg.ngroup), np.uint8), stream)
+    pw = d_pow.download(np.empty((cfg.nout, NBEAM, cfg.ngroup), np.float32), stream)
+    plan.close()
+    for b in d_in + [d_scale, d_out, d_pow, d_bp]:
+        b.free()
+    return dict(ms_per_step=total_ms / steps, kernel_ms=float(np.mean(kern_ms)),
+                kernel_ms_min=float(np.min(kern_ms)), out=out, power=pw, scale=scale)
+
+
+def errors(orc, cfg, block, W, res, groups):
+    d = orc.dims(NANT, NCHAN, NPKT, NBEAM, TAVG)
+    ref = orc.power(d, 1, block, W, 0, groups[0], groups[1])
+    sl = slice(groups[0], groups[1])
+    r, p = ref[:, :, sl], res["power"][:, :, sl].astype(np.float64)
+    rel = np.abs(p - r) / np.abs(r)
+    ref_u8 = orc.requant_block(ref, res["scale"])[:, :, sl].astype(np.int64)
+    du = np.abs(res["out"][:, :, sl].astype(np.int64) - ref_u8)
+    du = np.minimum(du, 256 - du)          # distance modulo the uint8 wrap
+    return dict(max_rel_power_err=float(rel.max()), rms_rel_power_err=float(np.sqrt((rel ** 2).mean())),
+                frac_u8_diff_ge1=float((du >= 1).mean()), frac_u8_diff_gt1=float((du > 1).mean()),
+                groups_checked=f"{groups[0]}..{groups[1] - 1}", mean_u8=float(res["out"].mean()))
+
+
+def cpu_baseline(orc, cfg, block, W, scale, threads, target_s=10.0):
+    d = orc.dims(NANT, NCHAN, NPKT, NBEAM, TAVG)
+    secs, reps = 0.0, 0
+    while secs < target_s and reps < 40:
+        s, _, _ = orc.cpu_beamform(d, 1, block, W, scale, 0, cfg.ngroup, threads)
+        secs += s
+        reps += 1
+    return dict(value=reps * SAMPLES_PER_BLOCK / secs, unit="samples/s", cores=threads, kind="port",
+                sample=f"{reps} full block(s) of the same workload, FP32 pthreads C loop "
+                       f"(oracle/bf_cpu.c), {secs:.1f} s total",
+                gflops=reps * FLOP_PER_BLOCK / secs / 1e9, s_per_block=secs / reps)
+
+
+def traffic_from_profiles(prec):
+    """HBM bytes per launch of the fused kernel from the committed rocprof
+    counter summary, or None when it has not been measured."""
+    path = os.path.join(ROOT, "profiles", "bf_fused_full_block_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"TRAFFIC_BYTES {prec} "):
+            return float(line.split()[2])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", choices=["fp8split", "fp16"], default="fp8split")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-check", action="store_true", help="skip the oracle error report")
+    ap.add_argument("--check-groups", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(channels shard across GPUs with no exchange; see the sharding note)", flush=True)
+        sys.exit(2)
+
+    import dsaxbf
+    ev = HipEvents()
+    t0 = time.time()
+    cfg, antpos, cal, theta, freqs, blocks, tone_beam = workload(dsaxbf, 3)
+    print(f"# workload ready in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    res = {}
+    for prec in ("fp16", "fp8split"):
+        res[prec] = time_path(dsaxbf, ev, prec, cfg, antpos, cal, theta, freqs, blocks,
+                              args.steps, args.warmup)
+        print(f"# {prec}: {res[prec]['ms_per_step']:.4f} ms/step, kernel "
+              f"{res[prec]['kernel_ms']:.4f} ms", file=sys.stderr, flush=True)
+
+    errs = {}
+    W = None
+    if not args.no_check or not args.no_cpu:
+        import orc
+        d = orc.dims(NANT, NCHAN, NPKT, NBEAM, TAVG)
+        W = orc.weights(d, 1, 1, antpos, cal, freqs, theta)
+    if not args.no_check:
+        g = (0, min(args.check_groups, cfg.ngroup))
+        for prec in res:
+            errs[prec] = errors(orc, cfg, blocks[0], W, res[prec], g)
+            print(f"# {prec} errors: {errs[prec]}", file=sys.stderr, flush=True)
+        peak = res["fp16"]["power"].astype(np.float64).sum(axis=(0, 2)).argmax()
+        errs["tone_beam"] = tone_beam
+        errs["peak_beam_fp16"] = int(peak)
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(orc, cfg, blocks[0], W, res["fp16"]["scale"], threads)
+        print(f"# cpu: {cpu}", file=sys.stderr, flush=True)
+
+    main_p = args.precision
+    r = res[main_p]
+    value = SAMPLES_PER_BLOCK / (r["ms_per_step"] / 1e3)
+    achieved = FLOP_PER_BLOCK / (r["kernel_ms"] / 1e3) / 1e12
+    line = {
+        "metric": METRIC, "value": value, "unit": "samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp8-e4m3 voltages x hi+lo fp8-e4m3 weights, fp32 accumulate"
+                 if main_p == "fp8split" else "fp16 x fp16, fp32 accumulate",
+        "data": "synthetic: seeded 4-bit complex Gaussian sigma 2 clipped +-7, tone SNR 10 in 4 "
+                "channels; geometry-derived FP64 weights (random-init equivalent)",
+        "config": {"workload": "full DSA block: 64 ant x 2 pol x 384 fine ch x 4096 samples "
+                               "(packed 4-bit, capture order) -> 256 beams, fine-channel "
+                               "weights, tavg 32 -> [128][256][48] uint8",
+                   "precision": main_p, "blocks_in_rotation": len(blocks),
+                   "realtime_fraction": r["ms_per_step"] / 134.217728},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[main_p],
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[main_p],
+                     "traffic": traffic_from_profiles(main_p),
+                     "kernel": "bf_fused_kernel", "kernel_ms": r["kernel_ms"],
+                     "flop_per_launch": FLOP_PER_BLOCK},
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
+                                                  ("value", "unit", "cores", "kind", "sample")},
+        "paths": {p: {"samples_per_s": SAMPLES_PER_BLOCK / (res[p]["ms_per_step"] / 1e3),
+                      "ms_per_step": res[p]["ms_per_step"], "kernel_ms": res[p]["kernel_ms"],
+                      "mfma_frac_of_own_peak": FLOP_PER_BLOCK / (res[p]["kernel_ms"] / 1e3) / 1e12
+                      / PEAK_TFLOPS[p],
+                      "executed_mfma_frac": (2.0 if p == "fp8split" else 1.0) * FLOP_PER_BLOCK
+                      / (res[p]["kernel_ms"] / 1e3) / 1e12 / PEAK_TFLOPS[p],
+                      "errors_vs_fp64_oracle": errs.get(p)} for p in res},
+        "fp8split_speedup_vs_fp16": res["fp16"]["ms_per_step"] / res["fp8split"]["ms_per_step"],
+        "tone_check": {k: errs[k] for k in ("tone_beam", "peak_beam_fp16") if k in errs},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -4,7 +4,7 @@
 // :267-416, badder :78-112, adder :120-158, calc_weights :420-449).  The
 // reference runs five launches per 32-sample "int" (640 per block) and keeps
 // unpacked FP16 planes and FP32 partial powers in HBM between them.  Here one
-// fused kernel does, per workgroup tile (one 8-channel output group x 64
+// fused kernel does, per workgroup tile (one 8-channel output group x TCU
 // samples x 256 beams):
 //   (a) coalesced 32-byte loads of the packed 4-bit block, transposed through
 //       LDS into [channel][pol][sample][antenna] rows (MFMA K-contiguous);
@@ -17,9 +17,15 @@
 // GEMM orientation: rows (M) = samples, columns (N) = beams, K = antennas x
 // {re, im}.  Each lane of a 32x32 accumulator holds one beam column and 16
 // sample rows, so the time average is an in-lane sum plus one cross-lane add.
-// The complex structure sits on the voltage side: with B-operand
-// K-slots [Wr, Wi], the real part uses A = [Dr, -Di] and the imaginary part
+// The complex structure sits on the voltage side: with B-operand K-slots
+// [Wr, Wi], the real part uses A = [Dr, -Di] and the imaginary part
 // A = [Di, Dr]; weights are stored once.
+//
+// What bounds it: every (channel, pol) weight fragment a wave loads feeds
+// only that wave's samples, so the weight stream from L2 (fine-channel
+// weights: 64 KB per (channel, pol) per 256 beams) sets the pace.  A
+// workgroup therefore owns BF_TH x 64 samples with BF_TH waves sharing each
+// weight fragment through L1.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <math.h>
@@ -214,12 +220,32 @@ __device__ __forceinline__ uint32_t nib_e4m3(uint32_t xs) {
 // ---------------------------------------------------------------------------
 // Fused beamformer.
 // ---------------------------------------------------------------------------
+// A workgroup of BF_BQ x BF_TH waves owns one output channel group g,
+// BF_TH x 64 samples and BF_BQ x BTW x 32 beams.  Wave w covers beam quarter
+// w % BF_BQ and time half w / BF_BQ; it holds the weights of its BTW beam
+// tiles for one (channel, pol) in registers and streams its staged rows
+// from LDS.  The BF_TH waves of a beam quarter load the same weight
+// fragments at about the same time, so L1 serves all but the first.
+#ifndef BF_BQ
+#define BF_BQ 4          // beam quarters (waves across beams)
+#endif
+#ifndef BF_TH
+#define BF_TH 1          // time halves (waves across samples)
+#endif
+#ifndef BF_BTW
+#define BF_BTW 2         // 32-beam tiles per wave
+#endif
+#ifndef BF_WPE
+#define BF_WPE 2         // waves per SIMD the register budget is sized for
+#endif
 namespace {
-constexpr int FT = 256;                 // threads: 4 waves, 64 beams each
-constexpr int TCU = 64;                 // samples per tile
-constexpr int PCU = TCU / 2;            // packets per tile
-constexpr int ROWB = 80;                // LDS bytes per (chan,pol,sample) row
+constexpr int TH = BF_TH, BQ = BF_BQ, BTW = BF_BTW;
+constexpr int NW = BQ * TH, NT = NW * 64;
+constexpr int TCU = 64 * TH;               // samples per tile
+constexpr int PCU = TCU / 2;               // packets per tile
+constexpr int ROWB = 80;                   // LDS bytes per (chan,pol,sample) row
 constexpr int LDS_BYTES = 16 * TCU * ROWB;
+constexpr int BSET = BQ * BTW * 32;        // beams per workgroup
 }
 
 struct BfArgs {
@@ -228,7 +254,8 @@ struct BfArgs {
   const float *wscale;
   const float *scale;
   uint8_t *out;
-  float *power;
+  float *power;   // optional
+  float *bpart;   // [nbeam][nsamp/64][ngroup] per-64-sample beam power sums
   int nant, nchan, ngroup, nbeam, nbeam_pad, nbt, wmode, order;
   int ntblk, nbsets, nwg_logical;
 };
@@ -245,34 +272,147 @@ __device__ __forceinline__ uint8_t requant_u8(float x, float scale) {
   return (uint8_t)(unsigned)(iv / 2);
 }
 
+// Weight fragments of one (channel, pol) for the wave's BTW beam tiles, and
+// the per-beam power scale that undoes the weight (and FP8 voltage) scaling.
+template <int PREC> struct WFrag;
+template <> struct WFrag<0> {
+  h8v w[BTW][8];
+  float sc[BTW];
+  __device__ __forceinline__ void load(const uint8_t *wfrag, const float *wscale, long cqrow,
+                                       long scrow, int lane, int n) {
+#pragma unroll
+    for (int bt = 0; bt < BTW; bt++)
+#pragma unroll
+      for (int s = 0; s < 8; s++)
+        w[bt][s] = *(const h8v *)(wfrag + (((cqrow + bt) * 8 + s) * 64 + lane) * 16);
+#pragma unroll
+    for (int bt = 0; bt < BTW; bt++) sc[bt] = wscale[scrow + bt * 32 + n];
+  }
+};
+template <> struct WFrag<1> {
+  i8v hi[BTW][2], lo[BTW][2];
+  float sc[BTW];
+  __device__ __forceinline__ void load(const uint8_t *wfrag, const float *wscale, long cqrow,
+                                       long scrow, int lane, int n) {
+#pragma unroll
+    for (int bt = 0; bt < BTW; bt++)
+#pragma unroll
+      for (int s2 = 0; s2 < 2; s2++) {
+        const long rec = (((cqrow + bt) * 2 + s2) * 2) * 64 + lane;
+        hi[bt][s2] = *(const i8v *)(wfrag + rec * 32);
+        lo[bt][s2] = *(const i8v *)(wfrag + (rec + 64) * 32);
+      }
+#pragma unroll
+    for (int bt = 0; bt < BTW; bt++) sc[bt] = wscale[scrow + bt * 32 + n];
+  }
+};
+
+// Sum of |B|^2 over the accumulator rows of each bin, two independent FMA
+// chains (real, imaginary) per bin, scaled into the partial power.
+template <int TAVG>
+__device__ __forceinline__ void detect_accumulate(const f16v (&ar)[BTW], const f16v (&ai)[BTW],
+                                                  const float (&sc)[BTW],
+                                                  float (&part)[BTW][TAVG <= 32 ? 32 / TAVG : 1]) {
+  constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;
+  constexpr int VPB = 16 / NP;
+#pragma unroll
+  for (int bt = 0; bt < BTW; bt++)
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+      float sr = 0.f, si = 0.f;
+#pragma unroll
+      for (int v = j * VPB; v < (j + 1) * VPB; v++) {
+        sr = __builtin_fmaf(ar[bt][v], ar[bt][v], sr);
+        si = __builtin_fmaf(ai[bt][v], ai[bt][v], si);
+      }
+      part[bt][j] = __builtin_fmaf(sr + si, sc[bt], part[bt][j]);
+    }
+}
+
+// One (channel, pol): the wave's two 32-sample row tiles against its weights;
+// detect and accumulate into the partial powers.
 template <int PREC, int TAVG>
-__global__ __launch_bounds__(FT) void bf_fused_kernel(BfArgs a) {
+__device__ __forceinline__ void beam_cq(const WFrag<PREC> &W, const uint8_t *rows, int n, int h,
+                                        float (&part)[TAVG == 64 ? 1 : 2][BTW]
+                                                     [TAVG <= 32 ? 32 / TAVG : 1]) {
+  constexpr int NTT = TAVG == 64 ? 1 : 2;
+  uint4 dv[2][2];
+#pragma unroll
+  for (int tt = 0; tt < 2; tt++) {
+    const uint8_t *src = rows + (tt * 32 + n) * ROWB + h * 32;
+    dv[tt][0] = *(const uint4 *)src;
+    dv[tt][1] = *(const uint4 *)(src + 16);
+  }
+#pragma unroll
+  for (int tt = 0; tt < 2; tt++) {
+    const uint32_t dw[8] = {dv[tt][0].x, dv[tt][0].y, dv[tt][0].z, dv[tt][0].w,
+                            dv[tt][1].x, dv[tt][1].y, dv[tt][1].z, dv[tt][1].w};
+    f16v ar[BTW], ai[BTW];
+#pragma unroll
+    for (int bt = 0; bt < BTW; bt++) { ar[bt] = (f16v){}; ai[bt] = (f16v){}; }
+    if constexpr (PREC == 0) {
+#pragma unroll
+      for (int s = 0; s < 8; s++) {
+        h8v are, aim;
+        unpack_f16(dw[s], are, aim);
+#pragma unroll
+        for (int bt = 0; bt < BTW; bt++) {
+          ar[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(are, W.w[bt][s], ar[bt], 0, 0, 0);
+          ai[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aim, W.w[bt][s], ai[bt], 0, 0, 0);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int s2 = 0; s2 < 2; s2++) {
+        i8v are, aim;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const uint32_t x = dw[4 * s2 + u];
+          const uint32_t dr = nib_e4m3(x), di = nib_e4m3(x >> 4);
+          are[2 * u] = (int)dr;
+          are[2 * u + 1] = (int)(di ^ 0x80808080u);
+          aim[2 * u] = (int)di;
+          aim[2 * u + 1] = (int)dr;
+        }
+#pragma unroll
+        for (int bt = 0; bt < BTW; bt++) {
+          ar[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.hi[bt][s2], ar[bt], 0, 0, 0, 0, 0, 0);
+          ar[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.lo[bt][s2], ar[bt], 0, 0, 0, 0, 0, 0);
+          ai[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.hi[bt][s2], ai[bt], 0, 0, 0, 0, 0, 0);
+          ai[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.lo[bt][s2], ai[bt], 0, 0, 0, 0, 0, 0);
+        }
+      }
+    }
+    detect_accumulate<TAVG>(ar, ai, W.sc, part[NTT == 1 ? 0 : tt]);
+  }
+}
+
+template <int PREC, int TAVG>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BF_WPE, BF_WPE)))
+void bf_fused_kernel(BfArgs a) {
   __shared__ __attribute__((aligned(16))) uint8_t lds[LDS_BYTES];
   constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;   // bins per 32-row tile
   constexpr int NTT = TAVG == 64 ? 1 : 2;          // tile-row groups with own bins
-  constexpr int VPB = 16 / NP;                     // accumulator values per bin
 
-  // XCD-aware tile order: workgroup P runs on XCD P % 8; consecutive logical
-  // tiles L go to the same XCD so the four groups sharing a 128-byte input
-  // line, and the tiles sharing a group's weights, meet in one L2.
+  // XCD-aware tile order: workgroup P runs on XCD P % 8 and logical tiles
+  // are dealt so that consecutive L run on one XCD.  Time blocks are the
+  // fastest index, so the workgroups resident on an XCD share one group's
+  // weights (16 (channel,pol) x 64 KB with fine weights) in its L2.
   const int nwg = gridDim.x;
   const int P = blockIdx.x;
   const int L = (P & 7) * (nwg >> 3) + (P >> 3);
   if (L >= a.nwg_logical) return;
-  const int gi = L & 3;
-  int rest = L >> 2;
-  const int bset = rest % a.nbsets;
-  rest /= a.nbsets;
-  const int tblk = rest % a.ntblk;
-  const int g = (rest / a.ntblk) * 4 + gi;
-  if (g >= a.ngroup) return;
+  const int tblk = L % a.ntblk;
+  const int rest = L / a.ntblk;
+  const int g = rest % a.ngroup;
+  const int bset = rest / a.ngroup;
 
   const int tid = threadIdx.x;
   const int p0 = tblk * PCU;
 
   // (a) stage: 32-byte chunk per (packet, antenna) = 8 channels x 2 t x 2 pol,
   // four antennas per task, 4x4 byte transposes, one dword per LDS row.
-  for (int task = tid; task < PCU * 16; task += FT) {
+  for (int task = tid; task < PCU * 16; task += NT) {
     const int pl = task >> 4, aq = task & 15;
     const int p = p0 + pl;
     uint32_t x[4][8];
@@ -298,222 +438,149 @@ __global__ __launch_bounds__(FT) void bf_fused_kernel(BfArgs a) {
       const uint32_t h01 = __builtin_amdgcn_perm(x[1][ci], x[0][ci], 0x07030602u);
       const uint32_t l23 = __builtin_amdgcn_perm(x[3][ci], x[2][ci], 0x05010400u);
       const uint32_t h23 = __builtin_amdgcn_perm(x[3][ci], x[2][ci], 0x07030602u);
-      const uint32_t o0 = __builtin_amdgcn_perm(l23, l01, 0x05040100u);  // t0 pol0
-      const uint32_t o1 = __builtin_amdgcn_perm(l23, l01, 0x07060302u);  // t0 pol1
-      const uint32_t o2 = __builtin_amdgcn_perm(h23, h01, 0x05040100u);  // t1 pol0
-      const uint32_t o3 = __builtin_amdgcn_perm(h23, h01, 0x07060302u);  // t1 pol1
       uint8_t *base = lds + aq * 4;
-      *(uint32_t *)(base + ((ci * 2 + 0) * TCU + 2 * pl) * ROWB) = o0;
-      *(uint32_t *)(base + ((ci * 2 + 1) * TCU + 2 * pl) * ROWB) = o1;
-      *(uint32_t *)(base + ((ci * 2 + 0) * TCU + 2 * pl + 1) * ROWB) = o2;
-      *(uint32_t *)(base + ((ci * 2 + 1) * TCU + 2 * pl + 1) * ROWB) = o3;
+      *(uint32_t *)(base + ((ci * 2 + 0) * TCU + 2 * pl) * ROWB) = __builtin_amdgcn_perm(l23, l01, 0x05040100u);
+      *(uint32_t *)(base + ((ci * 2 + 1) * TCU + 2 * pl) * ROWB) = __builtin_amdgcn_perm(l23, l01, 0x07060302u);
+      *(uint32_t *)(base + ((ci * 2 + 0) * TCU + 2 * pl + 1) * ROWB) = __builtin_amdgcn_perm(h23, h01, 0x05040100u);
+      *(uint32_t *)(base + ((ci * 2 + 1) * TCU + 2 * pl + 1) * ROWB) = __builtin_amdgcn_perm(h23, h01, 0x07060302u);
     }
   }
   __syncthreads();
 
   const int lane = tid & 63, wave = tid >> 6;
   const int n = lane & 31, h = lane >> 5;
-  const int btw = bset * 8 + wave * 2;     // first beam tile of this wave
+  const int bq = wave % BQ, th = wave / BQ;
+  const int btw = bset * (BSET / 32) + bq * BTW;   // first beam tile of this wave
 
-  float part[NTT][2][NP];
+  float part[NTT][BTW][NP];
 #pragma unroll
   for (int i = 0; i < NTT; i++)
 #pragma unroll
-    for (int bt = 0; bt < 2; bt++)
+    for (int bt = 0; bt < BTW; bt++)
 #pragma unroll
       for (int j = 0; j < NP; j++) part[i][bt][j] = 0.f;
 
+  // (b) 16 (channel, pol) GEMMs; cq = 2*channel + pol indexes the staged rows.
   for (int cq = 0; cq < 16; cq++) {
-    const int ci = cq >> 1, q = cq & 1;
-    const int c = g * 8 + ci;
-    const int wc = a.wmode ? c : g;
-    const long wrow = (long)(wc * 2 + q) * a.nbt + btw;   // beam-tile record row
-    float wsc[2];
-#pragma unroll
-    for (int bt = 0; bt < 2; bt++)
-      wsc[bt] = a.wscale[(long)(wc * 2 + q) * a.nbeam_pad + (btw + bt) * 32 + n];
-
-    if constexpr (PREC == 0) {
-      h8v B[2][8];
-#pragma unroll
-      for (int bt = 0; bt < 2; bt++)
-#pragma unroll
-        for (int s = 0; s < 8; s++)
-          B[bt][s] = *(const h8v *)(a.wfrag + (((wrow + bt) * 8 + s) * 64 + lane) * 16);
-#pragma unroll
-      for (int tt = 0; tt < 2; tt++) {
-        const uint8_t *src = lds + (cq * TCU + tt * 32 + n) * ROWB + h * 32;
-        const uint4 d0 = *(const uint4 *)src, d1 = *(const uint4 *)(src + 16);
-        const uint32_t dw[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
-        f16v ar[2], ai[2];
-#pragma unroll
-        for (int bt = 0; bt < 2; bt++) { ar[bt] = (f16v){}; ai[bt] = (f16v){}; }
-#pragma unroll
-        for (int s = 0; s < 8; s++) {
-          h8v are, aim;
-          unpack_f16(dw[s], are, aim);
-#pragma unroll
-          for (int bt = 0; bt < 2; bt++) {
-            ar[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(are, B[bt][s], ar[bt], 0, 0, 0);
-            ai[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aim, B[bt][s], ai[bt], 0, 0, 0);
-          }
-        }
-#pragma unroll
-        for (int bt = 0; bt < 2; bt++)
-#pragma unroll
-          for (int j = 0; j < NP; j++) {
-            float sacc = 0.f;
-#pragma unroll
-            for (int v = j * VPB; v < (j + 1) * VPB; v++)
-              sacc += ar[bt][v] * ar[bt][v] + ai[bt][v] * ai[bt][v];
-            part[NTT == 1 ? 0 : tt][bt][j] += sacc * wsc[bt];
-          }
-      }
-    } else {
-      i8v Bh[2][2], Bl[2][2];
-#pragma unroll
-      for (int bt = 0; bt < 2; bt++)
-#pragma unroll
-        for (int s2 = 0; s2 < 2; s2++) {
-          const long rec = (((wrow + bt) * 2 + s2) * 2) * 64 + lane;
-          Bh[bt][s2] = *(const i8v *)(a.wfrag + rec * 32);
-          Bl[bt][s2] = *(const i8v *)(a.wfrag + (rec + 64) * 32);
-        }
-#pragma unroll
-      for (int tt = 0; tt < 2; tt++) {
-        const uint8_t *src = lds + (cq * TCU + tt * 32 + n) * ROWB + h * 32;
-        const uint4 d0 = *(const uint4 *)src, d1 = *(const uint4 *)(src + 16);
-        const uint32_t dw[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
-        f16v ar[2], ai[2];
-#pragma unroll
-        for (int bt = 0; bt < 2; bt++) { ar[bt] = (f16v){}; ai[bt] = (f16v){}; }
-#pragma unroll
-        for (int s2 = 0; s2 < 2; s2++) {
-          i8v are, aim;
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            const uint32_t x = dw[4 * s2 + u];
-            const uint32_t dr = nib_e4m3(x), di = nib_e4m3(x >> 4);
-            are[2 * u] = (int)dr;
-            are[2 * u + 1] = (int)(di ^ 0x80808080u);
-            aim[2 * u] = (int)di;
-            aim[2 * u + 1] = (int)dr;
-          }
-#pragma unroll
-          for (int bt = 0; bt < 2; bt++) {
-            ar[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, Bh[bt][s2], ar[bt], 0, 0, 0, 0, 0, 0);
-            ar[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, Bl[bt][s2], ar[bt], 0, 0, 0, 0, 0, 0);
-            ai[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, Bh[bt][s2], ai[bt], 0, 0, 0, 0, 0, 0);
-            ai[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, Bl[bt][s2], ai[bt], 0, 0, 0, 0, 0, 0);
-          }
-        }
-#pragma unroll
-        for (int bt = 0; bt < 2; bt++)
-#pragma unroll
-          for (int j = 0; j < NP; j++) {
-            float sacc = 0.f;
-#pragma unroll
-            for (int v = j * VPB; v < (j + 1) * VPB; v++)
-              sacc += ar[bt][v] * ar[bt][v] + ai[bt][v] * ai[bt][v];
-            part[NTT == 1 ? 0 : tt][bt][j] += sacc * wsc[bt];
-          }
-      }
-    }
+    const long wr = (long)((a.wmode ? g * 8 + (cq >> 1) : g) * 2 + (cq & 1));
+    WFrag<PREC> W;
+    W.load(a.wfrag, a.wscale, wr * a.nbt + btw, wr * a.nbeam_pad + btw * 32, lane, n);
+    beam_cq<PREC, TAVG>(W, lds + (cq * TCU + th * 64) * ROWB, n, h, part);
   }
 
-  // (c) lanes h=0 and h=1 hold the two row halves of each bin.
+  // (c) lanes h=0 and h=1 hold the two row halves of each bin.  Besides the
+  // output, each wave leaves the sum of its bins per beam (the AGC input,
+  // badder in the reference) in bpart[beam][64-sample block][group].
+  const int t64 = tblk * TH + th;                  // this wave's 64-sample block
 #pragma unroll
-  for (int tt = 0; tt < NTT; tt++)
+  for (int bt = 0; bt < BTW; bt++) {
+    const int b = (btw + bt) * 32 + n;
+    float tile_sum = 0.f;
 #pragma unroll
-    for (int bt = 0; bt < 2; bt++)
+    for (int tt = 0; tt < NTT; tt++)
 #pragma unroll
       for (int j = 0; j < NP; j++) {
         float x = part[tt][bt][j];
         x += __shfl_xor(x, 32, 64);
-        const int b = (btw + bt) * 32 + n;
+        tile_sum += x;
         if (h == 0 && b < a.nbeam) {
-          const int o = TAVG == 64 ? tblk : tblk * (TCU / TAVG) + tt * (32 / TAVG) + j;
+          const int o = TAVG == 64 ? t64 : t64 * (64 / TAVG) + tt * (32 / TAVG) + j;
           const long idx = ((long)o * a.nbeam + b) * a.ngroup + g;
           if (a.power) a.power[idx] = x;
           a.out[idx] = requant_u8(x, a.scale[b]);
         }
       }
+    if (h == 0 && b < a.nbeam) a.bpart[((long)b * (a.ntblk * TH) + t64) * a.ngroup + g] = tile_sum;
+  }
 }
 
 // ---------------------------------------------------------------------------
-// -s / -q outputs: per (output bin, group) workgroup, one thread per antenna
-// sums |d|^2 over the bin's packets, 8 channels, 2 times and 2 pols.
+// -s / -q outputs: one workgroup per (64-sample block, group), one thread
+// per antenna sums |d|^2 per output bin over the block's packets, 8 channels,
+// 2 times and 2 pols; integer-valued sums are exact in float.
 //  stuff 1: beam b <- antenna ((b/16)%4)*16 + b%16  (dsaX_beamformer.cu:326-337)
 //  stuff 2: beam 0 <- sum over all antennas          (dsaX_beamformer.cu:307-323)
+// The per-block beam sums in bpart are rewritten for the beams written here.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void bf_stuff_kernel(bfk_dims d, const uint8_t *__restrict__ in,
                                                        const float *__restrict__ scale,
                                                        uint8_t *__restrict__ out,
-                                                       float *__restrict__ power) {
-  __shared__ float S[256];
-  __shared__ float red[4];
-  const int o = blockIdx.x / d.ngroup, g = blockIdx.x % d.ngroup;
+                                                       float *__restrict__ power,
+                                                       float *__restrict__ bpart) {
+  __shared__ float S[256][8];
+  __shared__ float IB[8];
+  const int nblk = d.nsamp / 64;
+  const int t64 = blockIdx.x % nblk, g = blockIdx.x / nblk;
+  const int nbin = 64 / d.tavg < 1 ? 1 : 64 / d.tavg;
+  const int ppb = d.tavg / 2 < 32 ? d.tavg / 2 : 32;      // packets per bin
   const int tid = threadIdx.x;
-  float s = 0.f;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if (tid < d.nant) {
-    for (int pl = 0; pl < d.tavg / 2; pl++) {
-      const int p = o * (d.tavg / 2) + pl;
-      long off = d.order == 0 ? (((long)p * d.nant + tid) * d.nchan + g * 8) * 4
-                              : (((long)p * d.ngroup + g) * d.nant + tid) * 32;
+    for (int pl = 0; pl < 32; pl++) {
+      const long p = (long)t64 * 32 + pl;
+      const long off = d.order == 0 ? ((p * d.nant + tid) * d.nchan + g * 8) * 4
+                                    : ((p * d.ngroup + g) * d.nant + tid) * 32;
       const uint4 *src = (const uint4 *)(in + off);
-      uint4 v[2] = {src[0], src[1]};
-      const uint32_t *w = (const uint32_t *)v;
+      const uint4 v0 = src[0], v1 = src[1];
+      const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+      float acc = 0.f;
+#pragma unroll
       for (int k = 0; k < 8; k++)
+#pragma unroll
         for (int by = 0; by < 4; by++) {
-          uint32_t bv = (w[k] >> (8 * by)) & 0xFFu;
-          int lo = bv & 15, hi = bv >> 4;
-          int re = lo >= 8 ? lo - 16 : lo, im = hi >= 8 ? hi - 16 : hi;
-          s += (float)(re * re + im * im);
+          const uint32_t bv = (w[k] >> (8 * by)) & 0xFFu;
+          const int lo = bv & 15, hi = bv >> 4;
+          const int re = lo >= 8 ? lo - 16 : lo, im = hi >= 8 ? hi - 16 : hi;
+          acc += (float)(re * re + im * im);
         }
+      const int bin = pl / ppb;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (k == bin) s[k] += acc;
     }
   }
-  S[tid] = s;
-  // block sum for the incoherent beam (exact: integer-valued floats < 2^24)
-  float w = s;
-  for (int m = 32; m >= 1; m >>= 1) w += __shfl_xor(w, m, 64);
-  if ((tid & 63) == 0) red[tid >> 6] = w;
+#pragma unroll
+  for (int k = 0; k < 8; k++) S[tid][k] = s[k];
   __syncthreads();
-  if (d.stuff == 2) {
-    if (tid == 0) {
-      float x = red[0] + red[1] + red[2] + red[3];
-      long idx = ((long)o * d.nbeam + 0) * d.ngroup + g;
-      power[idx] = x;
-      out[idx] = requant_u8(x, scale[0]);
-    }
-  } else {
-    for (int b = tid; b < d.nbeam; b += 256) {
-      int ant = ((b / 16) % 4) * 16 + b % 16;
-      float x = ant < d.nant ? S[ant] : 0.f;
-      long idx = ((long)o * d.nbeam + b) * d.ngroup + g;
-      power[idx] = x;
+  if (tid < nbin) {
+    float x = 0.f;
+    for (int a = 0; a < d.nant; a++) x += S[a][tid];
+    IB[tid] = x;
+  }
+  __syncthreads();
+  const int bend = d.stuff == 2 ? 1 : d.nbeam;
+  for (int b = tid; b < bend; b += 256) {
+    const int ant = ((b / 16) % 4) * 16 + b % 16;
+    float tile = 0.f;
+    for (int k = 0; k < nbin; k++) {
+      const float x = d.stuff == 2 ? IB[k] : (ant < d.nant ? S[ant][k] : 0.f);
+      const long idx = ((long)(t64 * nbin + k) * d.nbeam + b) * d.ngroup + g;
+      if (power) power[idx] = x;
       out[idx] = requant_u8(x, scale[b]);
+      tile += x;
     }
+    bpart[((long)b * nblk + t64) * d.ngroup + g] = tile;
   }
 }
 
-// beam_power[b] = sum_{o,g} power[o][b][g]   (badder + calc_allbp)
-__global__ __launch_bounds__(256) void bf_beam_power_kernel(bfk_dims d, const float *__restrict__ power,
+// beam_power[b] = sum over 64-sample blocks and groups of bpart[b][.][.]
+// (badder + calc_allbp, dsaX_beamformer.cu:78-112,482-495).  One workgroup
+// per beam, coalesced reads, fixed-order FP64 tree: deterministic.
+__global__ __launch_bounds__(256) void bf_beam_power_kernel(bfk_dims d, const float *__restrict__ bpart,
                                                             float *__restrict__ bp) {
-  __shared__ float red[4];
+  __shared__ double red[256];
   const int b = blockIdx.x, tid = threadIdx.x;
+  const long n = (long)(d.nsamp / 64) * d.ngroup;
+  const float *src = bpart + (long)b * n;
   double s = 0.0;
-  const long n = (long)d.nout * d.ngroup;
-  for (long i = tid; i < n; i += 256) {
-    const long o = i / d.ngroup, g = i % d.ngroup;
-    s += power[(o * d.nbeam + b) * d.ngroup + g];
-  }
-  float w = (float)s;
-  double wd = s;
-  for (int m = 32; m >= 1; m >>= 1) wd += __shfl_xor(wd, m, 64);
-  if ((tid & 63) == 0) red[tid >> 6] = (float)wd;
+  for (long i = tid; i < n; i += 256) s += src[i];
+  red[tid] = s;
   __syncthreads();
-  (void)w;
-  if (tid == 0) bp[b] = red[0] + red[1] + red[2] + red[3];
+  for (int m = 128; m > 0; m >>= 1) {
+    if (tid < m) red[tid] += red[tid + m];
+    __syncthreads();
+  }
+  if (tid == 0) bp[b] = (float)red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -532,17 +599,17 @@ extern "C" int bfk_weights(const bfk_dims *d, const float *d_antpos, const float
 template <int PREC>
 static void launch_fused(int tavg, dim3 grid, hipStream_t s, const BfArgs &a) {
   switch (tavg) {
-    case 8:  hipLaunchKernelGGL((bf_fused_kernel<PREC, 8>), grid, dim3(FT), 0, s, a); break;
-    case 16: hipLaunchKernelGGL((bf_fused_kernel<PREC, 16>), grid, dim3(FT), 0, s, a); break;
-    case 32: hipLaunchKernelGGL((bf_fused_kernel<PREC, 32>), grid, dim3(FT), 0, s, a); break;
-    default: hipLaunchKernelGGL((bf_fused_kernel<PREC, 64>), grid, dim3(FT), 0, s, a); break;
+    case 8:  hipLaunchKernelGGL((bf_fused_kernel<PREC, 8>), grid, dim3(NT), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((bf_fused_kernel<PREC, 16>), grid, dim3(NT), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((bf_fused_kernel<PREC, 32>), grid, dim3(NT), 0, s, a); break;
+    default: hipLaunchKernelGGL((bf_fused_kernel<PREC, 64>), grid, dim3(NT), 0, s, a); break;
   }
 }
 
 extern "C" int bfk_beamform(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
                             const float *d_wscale, const float *d_scale, uint8_t *d_out,
-                            float *d_power, void *stream) {
-  if (d->nant_pad != 64 || d->nbeam_pad % 256 != 0 || d->nsamp % TCU != 0) {
+                            float *d_power, float *d_bpart, void *stream) {
+  if (d->nant_pad != 64 || d->nbeam_pad % BSET != 0 || d->nsamp % TCU != 0) {
     snprintf(g_err, sizeof(g_err), "bfk_beamform: unsupported shape nant_pad=%d nbeam_pad=%d nsamp=%d",
              d->nant_pad, d->nbeam_pad, d->nsamp);
     return -1;
@@ -554,6 +621,7 @@ extern "C" int bfk_beamform(const bfk_dims *d, const uint8_t *d_in, const void *
   a.scale = d_scale;
   a.out = d_out;
   a.power = d_power;
+  a.bpart = d_bpart;
   a.nant = d->nant;
   a.nchan = d->nchan;
   a.ngroup = d->ngroup;
@@ -563,9 +631,8 @@ extern "C" int bfk_beamform(const bfk_dims *d, const uint8_t *d_in, const void *
   a.wmode = d->wmode;
   a.order = d->order;
   a.ntblk = d->nsamp / TCU;
-  a.nbsets = d->nbeam_pad / 256;
-  const int nquad = (d->ngroup + 3) / 4;
-  a.nwg_logical = nquad * 4 * a.ntblk * a.nbsets;
+  a.nbsets = d->nbeam_pad / BSET;
+  a.nwg_logical = a.nbsets * d->ngroup * a.ntblk;
   const int nwg = (a.nwg_logical + 7) / 8 * 8;
   hipStream_t s = (hipStream_t)stream;
   if (d->prec == 0) launch_fused<0>(d->tavg, dim3(nwg), s, a);
@@ -574,19 +641,23 @@ extern "C" int bfk_beamform(const bfk_dims *d, const uint8_t *d_in, const void *
 }
 
 extern "C" int bfk_stuff(const bfk_dims *d, const uint8_t *d_in, const float *d_scale,
-                         uint8_t *d_out, float *d_power, void *stream) {
-  if (d->nant > 256) {
-    snprintf(g_err, sizeof(g_err), "bfk_stuff: nant > 256");
+                         uint8_t *d_out, float *d_power, float *d_bpart, void *stream) {
+  if (d->nant > 256 || d->nsamp % 64 != 0) {
+    snprintf(g_err, sizeof(g_err), "bfk_stuff: unsupported shape nant=%d nsamp=%d", d->nant, d->nsamp);
     return -1;
   }
-  hipLaunchKernelGGL(bf_stuff_kernel, dim3(d->nout * d->ngroup), dim3(256), 0, (hipStream_t)stream,
-                     *d, d_in, d_scale, d_out, d_power);
+  hipLaunchKernelGGL(bf_stuff_kernel, dim3((d->nsamp / 64) * d->ngroup), dim3(256), 0,
+                     (hipStream_t)stream, *d, d_in, d_scale, d_out, d_power, d_bpart);
   return bfk_check(hipGetLastError(), "bf_stuff_kernel");
 }
 
-extern "C" int bfk_beam_power(const bfk_dims *d, const float *d_power, float *d_beam_power,
+extern "C" size_t bfk_bpart_bytes(const bfk_dims *d) {
+  return (size_t)d->nbeam * (d->nsamp / 64) * d->ngroup * sizeof(float);
+}
+
+extern "C" int bfk_beam_power(const bfk_dims *d, const float *d_bpart, float *d_beam_power,
                               void *stream) {
   hipLaunchKernelGGL(bf_beam_power_kernel, dim3(d->nbeam), dim3(256), 0, (hipStream_t)stream, *d,
-                     d_power, d_beam_power);
+                     d_bpart, d_beam_power);
   return bfk_check(hipGetLastError(), "bf_beam_power_kernel");
 }

@@ -43,17 +43,23 @@ int bfk_weights(const bfk_dims *d, const float *d_antpos, const float *d_cal,
                 const double *d_freqs, const double *d_theta, void *d_wfrag,
                 float *d_wscale, void *stream);
 
-/* Fused unpack -> MFMA beam GEMM -> |B|^2 -> average -> requantise. */
+/* Per-tile beam power sums written by every beamforming launch:
+ * [nbeam][nsamp/64][ngroup] floats. */
+size_t bfk_bpart_bytes(const bfk_dims *d);
+
+/* Fused unpack -> MFMA beam GEMM -> |B|^2 -> average -> requantise.
+ * d_power is optional; d_bpart receives the per-tile beam power sums. */
 int bfk_beamform(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
                  const float *d_wscale, const float *d_scale, uint8_t *d_out,
-                 float *d_power, void *stream);
+                 float *d_power, float *d_bpart, void *stream);
 
-/* Antenna-stuffing (-s) and incoherent-beam (-q) outputs. */
+/* Antenna-stuffing (-s) and incoherent-beam (-q) outputs (rewrites the
+ * d_bpart entries of the beams it writes). */
 int bfk_stuff(const bfk_dims *d, const uint8_t *d_in, const float *d_scale,
-              uint8_t *d_out, float *d_power, void *stream);
+              uint8_t *d_out, float *d_power, float *d_bpart, void *stream);
 
-/* beam_power[b] = sum over outputs and groups of power (deterministic). */
-int bfk_beam_power(const bfk_dims *d, const float *d_power, float *d_beam_power,
+/* beam_power[b] = sum of d_bpart[b][.][.] (deterministic order). */
+int bfk_beam_power(const bfk_dims *d, const float *d_bpart, float *d_beam_power,
                    void *stream);
 
 /* Name of the last HIP error seen by a launcher (thread-local). */

@@ -27,6 +27,7 @@ struct dsaxbf_plan {
   float *d_power;       /* internal power buffer (always available) */
   float *d_beam_power;
   float *d_scale;
+  float *d_bpart;       /* per-tile beam power sums of the last execute */
   int have_weights;
 };
 
@@ -70,6 +71,7 @@ void dsaxbf_destroy(dsaxbf_plan *p) {
   hipFree(p->d_power);
   hipFree(p->d_beam_power);
   hipFree(p->d_scale);
+  hipFree(p->d_bpart);
   if (p->stream) hipStreamDestroy(p->stream);
   free(p);
 }
@@ -103,6 +105,7 @@ int dsaxbf_create(dsaxbf_plan **out, const dsaxbf_config *cfg) {
   if (e == hipSuccess) e = hipMalloc((void **)&p->d_power, nout * sizeof(float));
   if (e == hipSuccess) e = hipMalloc((void **)&p->d_beam_power, (size_t)cfg->nbeam * sizeof(float));
   if (e == hipSuccess) e = hipMalloc((void **)&p->d_scale, (size_t)cfg->nbeam * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void **)&p->d_bpart, bfk_bpart_bytes(d));
   if (e != hipSuccess) {
     dsaxbf_destroy(p);
     return dsaxbf_fail(DSAXBF_ENOMEM, "device allocation failed: %s", hipGetErrorString(e));
@@ -167,21 +170,26 @@ int dsaxbf_execute(dsaxbf_plan *p, const uint8_t *d_in, const float *d_scale, ui
   if (!p->have_weights && d->stuff != DSAXBF_STUFF_ANTS)
     return dsaxbf_fail(DSAXBF_ESTATE, "execute before dsaxbf_set_weights");
   void *s = stream ? stream : (void *)p->stream;
-  float *pw = d_power ? d_power : p->d_power;
   int rc;
   if (d->stuff != DSAXBF_STUFF_ANTS) {
-    rc = bfk_beamform(d, d_in, p->d_wfrag, p->d_wscale, d_scale, d_out, pw, s);
+    rc = bfk_beamform(d, d_in, p->d_wfrag, p->d_wscale, d_scale, d_out, d_power, p->d_bpart, s);
     if (rc) return dsaxbf_fail(rc, "%s", bfk_last_error());
   }
   if (d->stuff != DSAXBF_STUFF_NONE) {
-    rc = bfk_stuff(d, d_in, d_scale, d_out, pw, s);
+    rc = bfk_stuff(d, d_in, d_scale, d_out, d_power, p->d_bpart, s);
     if (rc) return dsaxbf_fail(rc, "%s", bfk_last_error());
   }
   if (d_beam_power) {
-    rc = bfk_beam_power(d, pw, d_beam_power, s);
+    rc = bfk_beam_power(d, p->d_bpart, d_beam_power, s);
     if (rc) return dsaxbf_fail(rc, "%s", bfk_last_error());
   }
   return DSAXBF_OK;
+}
+
+int dsaxbf_beam_power(dsaxbf_plan *p, float *d_beam_power, void *stream) {
+  if (!p || !d_beam_power) return dsaxbf_fail(DSAXBF_EINVAL, "NULL argument");
+  int rc = bfk_beam_power(&p->dims, p->d_bpart, d_beam_power, stream ? stream : (void *)p->stream);
+  return rc ? dsaxbf_fail(rc, "%s", bfk_last_error()) : DSAXBF_OK;
 }
 
 int dsaxbf_process_block(dsaxbf_plan *p, const uint8_t *h_in, const float *h_scale,
@@ -194,7 +202,7 @@ int dsaxbf_process_block(dsaxbf_plan *p, const uint8_t *h_in, const float *h_sca
   HIPCHK(hipMemcpyAsync(p->d_in, h_in, nin, hipMemcpyHostToDevice, s), "H2D block");
   HIPCHK(hipMemcpyAsync(p->d_scale, h_scale, sizeof(float) * p->cfg.nbeam, hipMemcpyHostToDevice, s),
          "H2D scale");
-  int rc = dsaxbf_execute(p, p->d_in, p->d_scale, p->d_out, p->d_power,
+  int rc = dsaxbf_execute(p, p->d_in, p->d_scale, p->d_out, h_power ? p->d_power : NULL,
                           h_beam_power ? p->d_beam_power : NULL, s);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(h_out, p->d_out, nout, hipMemcpyDeviceToHost, s), "D2H output");

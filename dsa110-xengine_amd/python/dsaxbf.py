@@ -18,7 +18,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libdsaxbf.so")
+LIB_PATH = os.environ.get("DSAXBF_LIB") or os.path.join(os.path.dirname(_HERE), "lib",
+                                                         "libdsaxbf.so")
 
 PREC_FP16 = 0
 PREC_FP8SPLIT = 1
@@ -89,6 +90,7 @@ def _load():
         "dsaxbf_reference_freqs": (None, [C.c_double, C.c_int, P]),
         "dsaxbf_execute": (C.c_int, [P, P, P, P, P, P, P]),
         "dsaxbf_process_block": (C.c_int, [P, P, P, P, P, P]),
+        "dsaxbf_beam_power": (C.c_int, [P, P, P]),
         "dsaxbf_agc_init": (C.c_int, [C.POINTER(AGC), C.c_int, C.c_int, C.c_int, C.c_int]),
         "dsaxbf_agc_free": (None, [C.POINTER(AGC)]),
         "dsaxbf_agc_update": (C.c_int, [C.POINTER(AGC), P, P, P]),
@@ -119,7 +121,7 @@ EXPORTED = [
     "dsaxbf_config_default", "dsaxbf_input_bytes", "dsaxbf_output_bytes", "dsaxbf_last_error",
     "dsaxbf_create", "dsaxbf_destroy", "dsaxbf_get_config", "dsaxbf_get_stream",
     "dsaxbf_read_weights_file", "dsaxbf_set_weights", "dsaxbf_reference_freqs",
-    "dsaxbf_execute", "dsaxbf_process_block", "dsaxbf_agc_init", "dsaxbf_agc_free",
+    "dsaxbf_execute", "dsaxbf_beam_power", "dsaxbf_process_block", "dsaxbf_agc_init", "dsaxbf_agc_free",
     "dsaxbf_agc_update", "dsaxbf_requant", "dsaxbf_synth_block", "dsaxbf_synth_geometry",
     "dsaxbf_dev_alloc", "dsaxbf_dev_free", "dsaxbf_copy_h2d", "dsaxbf_copy_d2h", "dsaxbf_sync",
     "dsaxbf_antpol_rms", "dsaxbf_version",
@@ -267,6 +269,11 @@ class Plan:
             return None if x is None else (x.ptr if isinstance(x, DeviceBuffer) else C.c_void_p(x))
         _check(lib.dsaxbf_execute(self._h, p(d_in), p(d_scale), p(d_out), p(d_power),
                                   p(d_beam_power), stream))
+
+    def beam_power(self, d_beam_power, stream=None):
+        """Per-beam power sum of the last execute (device buffer of nbeam floats)."""
+        p = d_beam_power.ptr if isinstance(d_beam_power, DeviceBuffer) else C.c_void_p(d_beam_power)
+        _check(lib.dsaxbf_beam_power(self._h, p, stream))
 
     def close(self):
         if self._h:

@@ -169,6 +169,13 @@ int dsaxbf_execute(dsaxbf_plan *plan, const uint8_t *d_in, const float *d_scale,
                    uint8_t *d_out, float *d_power, float *d_beam_power,
                    void *stream);
 
+/* Per-beam power sum of the block most recently run by dsaxbf_execute on
+ * this plan: d_beam_power[b] = sum over outputs and groups of the power
+ * (badder + calc_allbp, dsaX_beamformer.cu:78-112,482-495), from partial
+ * sums the beamforming kernel leaves in the plan.  dsaxbf_execute does this
+ * itself when given d_beam_power.  Same stream as that execute. */
+int dsaxbf_beam_power(dsaxbf_plan *plan, float *d_beam_power, void *stream);
+
 /* Same as dsaxbf_execute with host pointers: copies the block in, runs it,
  * copies the output (and optional power / beam_power) back, then waits.
  * h_scale is [nbeam] floats on the host.  This is the psrdada-facing call
