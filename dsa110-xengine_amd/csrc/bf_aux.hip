@@ -1,0 +1,111 @@
+// bf_aux.hip -- the small kernels around the fused beamformer: antenna
+// stuffing (-s) and the incoherent beam (-q), and the per-beam power sums
+// the AGC consumes.
+#include "bf_common.h"
+
+// One workgroup per (64-sample block, group), one thread per antenna sums
+// |d|^2 per output bin over the block's packets, 8 channels, 2 times and 2
+// pols; integer-valued sums are exact in float.
+//  stuff 1: beam b <- antenna ((b/16)%4)*16 + b%16  (dsaX_beamformer.cu:326-337)
+//  stuff 2: beam 0 <- sum over all antennas          (dsaX_beamformer.cu:307-323)
+// The per-block beam sums in bpart are rewritten for the beams written here.
+__global__ __launch_bounds__(256) void bf_stuff_kernel(bfk_dims d, const uint8_t *__restrict__ in,
+                                                       const float *__restrict__ scale,
+                                                       uint8_t *__restrict__ out,
+                                                       float *__restrict__ power,
+                                                       float *__restrict__ bpart) {
+  __shared__ float S[256][8];
+  __shared__ float IB[8];
+  const int nblk = d.nsamp / 64;
+  const int t64 = blockIdx.x % nblk, g = blockIdx.x / nblk;
+  const int nbin = 64 / d.tavg < 1 ? 1 : 64 / d.tavg;
+  const int ppb = d.tavg / 2 < 32 ? d.tavg / 2 : 32;      // packets per bin
+  const int tid = threadIdx.x;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (tid < d.nant) {
+    for (int pl = 0; pl < 32; pl++) {
+      const long p = (long)t64 * 32 + pl;
+      const uint4 *src = (const uint4 *)(in + chunk_offset(d.order, p, tid, g, d.nant, d.nchan, d.ngroup));
+      const uint4 v0 = src[0], v1 = src[1];
+      const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+#pragma unroll
+        for (int by = 0; by < 4; by++) {
+          const uint32_t bv = (w[k] >> (8 * by)) & 0xFFu;
+          const int lo = bv & 15, hi = bv >> 4;
+          const int re = lo >= 8 ? lo - 16 : lo, im = hi >= 8 ? hi - 16 : hi;
+          acc += (float)(re * re + im * im);
+        }
+      const int bin = pl / ppb;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (k == bin) s[k] += acc;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; k++) S[tid][k] = s[k];
+  __syncthreads();
+  if (tid < nbin) {
+    float x = 0.f;
+    for (int a = 0; a < d.nant; a++) x += S[a][tid];
+    IB[tid] = x;
+  }
+  __syncthreads();
+  const int bend = d.stuff == 2 ? 1 : d.nbeam;
+  for (int b = tid; b < bend; b += 256) {
+    const int ant = ((b / 16) % 4) * 16 + b % 16;
+    float tile = 0.f;
+    for (int k = 0; k < nbin; k++) {
+      const float x = d.stuff == 2 ? IB[k] : (ant < d.nant ? S[ant][k] : 0.f);
+      const long idx = ((long)(t64 * nbin + k) * d.nbeam + b) * d.ngroup + g;
+      if (power) power[idx] = x;
+      out[idx] = requant_u8(x, scale[b]);
+      tile += x;
+    }
+    bpart[((long)b * nblk + t64) * d.ngroup + g] = tile;
+  }
+}
+
+// beam_power[b] = sum over 64-sample blocks and groups of bpart[b][.][.]
+// (badder + calc_allbp, dsaX_beamformer.cu:78-112,482-495).  One workgroup
+// per beam, coalesced reads, fixed-order FP64 tree: deterministic.
+__global__ __launch_bounds__(256) void bf_beam_power_kernel(bfk_dims d, const float *__restrict__ bpart,
+                                                            float *__restrict__ bp) {
+  __shared__ double red[256];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const long n = (long)(d.nsamp / 64) * d.ngroup;
+  const float *src = bpart + (long)b * n;
+  double s = 0.0;
+  for (long i = tid; i < n; i += 256) s += src[i];
+  red[tid] = s;
+  __syncthreads();
+  for (int m = 128; m > 0; m >>= 1) {
+    if (tid < m) red[tid] += red[tid + m];
+    __syncthreads();
+  }
+  if (tid == 0) bp[b] = (float)red[0];
+}
+
+extern "C" int bfk_stuff(const bfk_dims *d, const uint8_t *d_in, const float *d_scale,
+                         uint8_t *d_out, float *d_power, float *d_bpart, void *stream) {
+  if (d->nant > 256 || d->nsamp % 64 != 0) {
+    bfk_set_error("bfk_stuff: unsupported shape nant=%d nsamp=%d", d->nant, d->nsamp);
+    return -1;
+  }
+  hipLaunchKernelGGL(bf_stuff_kernel, dim3((d->nsamp / 64) * d->ngroup), dim3(256), 0,
+                     (hipStream_t)stream, *d, d_in, d_scale, d_out, d_power, d_bpart);
+  return bfk_check(hipGetLastError(), "bf_stuff_kernel");
+}
+
+extern "C" size_t bfk_bpart_bytes(const bfk_dims *d) {
+  return (size_t)d->nbeam * (d->nsamp / 64) * d->ngroup * sizeof(float);
+}
+
+extern "C" int bfk_beam_power(const bfk_dims *d, const float *d_bpart, float *d_beam_power,
+                              void *stream) {
+  hipLaunchKernelGGL(bf_beam_power_kernel, dim3(d->nbeam), dim3(256), 0, (hipStream_t)stream, *d,
+                     d_bpart, d_beam_power);
+  return bfk_check(hipGetLastError(), "bf_beam_power_kernel");
+}
