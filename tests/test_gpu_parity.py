@@ -99,6 +99,25 @@ def test_stuff_modes_exact(bf, orc, stuff, tavg):
         assert rel.max() <= TOL[0]
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_small_block_golden(bf, orc, prec):
+    """The committed small block (64 ant x 48 ch x 1024 samples, 32 fan beams,
+    fine-channel sin(theta) weights, tavg 32, fixed scale 2^-11)."""
+    import sys
+    sys.path.insert(0, GOLDEN)
+    import make_golden
+    g = np.load(os.path.join(GOLDEN, "small_block.npz"), allow_pickle=False)
+    cfg, block, antpos, cal, freqs, theta, _, _ = make_golden.small_inputs()
+    cfg.precision = prec
+    plan = bf.Plan(cfg)
+    plan.set_weights(antpos, cal, freqs, theta)
+    out, pw, _ = plan.process_block(block, g["scale"], want_power=True)
+    plan.close()
+    r = dict(ref=g["power"], power=pw, out=out, ref_u8=g["out"])
+    rel, du = _check(r, prec)
+    assert (du > 0).mean() <= 1e-3
+
+
 def test_beam_power_matches_oracle(bf, orc):
     r = _run(bf, orc, prec=0, tavg=16, seed=3)
     bp = orc.beam_power(r["d"], orc.power(r["d"], 1, r["block"], r["W"]))
