@@ -83,10 +83,17 @@ def workload(dsaxbf, nblocks, seed=2024):
     return cfg, antpos, cal, theta, freqs, blocks, tone_beam
 
 
-def time_path(dsaxbf, ev, prec_name, cfg, antpos, cal, theta, freqs, blocks, steps, warmup):
+def time_path(dsaxbf, ev, prec_name, cfg, antpos, cal, theta, freqs, blocks, steps, warmup,
+              schedule="auto"):
     cfg.precision = dsaxbf.PREC_FP16 if prec_name == "fp16" else dsaxbf.PREC_FP8SPLIT
     plan = dsaxbf.Plan(cfg)
     plan.set_weights(antpos, cal, freqs, theta)
+    tune_ms = None
+    if schedule == "auto":      # the library's autotuner picks the schedule
+        tune_ms = plan.autotune(5)
+    else:
+        plan.schedule = {v: k for k, v in dsaxbf.SCHED_NAMES.items()}[schedule]
+    sched_name = dsaxbf.SCHED_NAMES[plan.schedule]
     stream = plan.stream
     nout = cfg.nout * cfg.nbeam * cfg.ngroup
     d_in = [dsaxbf.DeviceBuffer(b.nbytes) for b in blocks]
@@ -133,7 +140,10 @@ def time_path(dsaxbf, ev, prec_name, cfg, antpos, cal, theta, freqs, blocks, ste
     for b in d_in + [d_scale, d_out, d_pow, d_bp]:
         b.free()
     return dict(ms_per_step=total_ms / steps, kernel_ms=float(np.mean(kern_ms)),
-                kernel_ms_min=float(np.min(kern_ms)), out=out, power=pw, scale=scale)
+                kernel_ms_min=float(np.min(kern_ms)), out=out, power=pw, scale=scale,
+                schedule=sched_name,
+                autotune_ms=None if tune_ms is None else
+                {dsaxbf.SCHED_NAMES[i]: float(t) for i, t in enumerate(tune_ms)})
 
 
 def errors(orc, cfg, block, W, res, groups):
@@ -184,6 +194,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the oracle error report")
     ap.add_argument("--check-groups", type=int, default=8)
+    ap.add_argument("--schedule", choices=["auto", "classic", "pipelined"], default="auto",
+                    help="fused-kernel schedule (auto: the library's autotuner picks)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy, three times: the "
                          "known-size kernel that scales the HBM byte counters")
@@ -211,9 +223,10 @@ def main():
     res = {}
     for prec in ("fp16", "fp8split"):
         res[prec] = time_path(dsaxbf, ev, prec, cfg, antpos, cal, theta, freqs, blocks,
-                              args.steps, args.warmup)
+                              args.steps, args.warmup, args.schedule)
         print(f"# {prec}: {res[prec]['ms_per_step']:.4f} ms/step, kernel "
-              f"{res[prec]['kernel_ms']:.4f} ms", file=sys.stderr, flush=True)
+              f"{res[prec]['kernel_ms']:.4f} ms, schedule {res[prec]['schedule']} "
+              f"(autotune ms {res[prec]['autotune_ms']})", file=sys.stderr, flush=True)
 
     errs = {}
     W = None
@@ -255,7 +268,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[main_p],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[main_p],
                      "traffic": traffic_from_profiles(main_p),
-                     "kernel": "bf_fused_kernel", "kernel_ms": r["kernel_ms"],
+                     "kernel": f"bf_{r['schedule']}_kernel", "kernel_ms": r["kernel_ms"],
                      "flop_per_launch": FLOP_PER_BLOCK},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
                                                   ("value", "unit", "cores", "kind", "sample")},
@@ -265,6 +278,7 @@ def main():
                       / PEAK_TFLOPS[p],
                       "executed_mfma_frac": (2.0 if p == "fp8split" else 1.0) * FLOP_PER_BLOCK
                       / (res[p]["kernel_ms"] / 1e3) / 1e12 / PEAK_TFLOPS[p],
+                      "schedule": res[p]["schedule"], "autotune_ms": res[p]["autotune_ms"],
                       "errors_vs_fp64_oracle": errs.get(p)} for p in res},
         "fp8split_speedup_vs_fp16": res["fp16"]["ms_per_step"] / res["fp8split"]["ms_per_step"],
         "tone_check": {k: errs[k] for k in ("tone_beam", "peak_beam_fp16") if k in errs},

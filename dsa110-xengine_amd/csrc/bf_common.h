@@ -110,4 +110,265 @@ __device__ __forceinline__ long chunk_offset(int order, long p, int a, int g, in
   return order == 0 ? ((p * nant + a) * nchan + g * 8) * 4 : ((p * ngroup + g) * nant + a) * 32;
 }
 
+// ---------------------------------------------------------------------------
+// The fused beamformer's tile.  A tile is one 8-channel output group g x 64
+// samples x a set of beams.  Its voltages are staged in LDS as rows
+// [channel*2 + pol][sample][antenna] of 64 packed bytes (4-bit re | 4-bit im
+// << 4), padded to ROWB bytes; row r of (channel, pol) cq sits at
+// (cq*TCU + r)*ROWB.  A wave owns BTW 32-beam tiles; for each (channel, pol)
+// and each 32-sample half of the tile (a "unit") it runs the complex GEMM on
+// MFMA with its weights in registers, then detects |B|^2 into per-bin sums.
+// ---------------------------------------------------------------------------
+constexpr int BTW = 2;                    // 32-beam tiles per wave
+constexpr int TCU = 64;                   // samples per tile
+constexpr int PCU = TCU / 2;              // packets per tile
+constexpr int ROWB = 80;                  // LDS bytes per staged row
+constexpr int STAGE_BYTES = 16 * TCU * ROWB;
+
+struct BfArgs {
+  const uint8_t *in;
+  const uint8_t *wfrag;
+  const float *wscale;
+  const float *scale;
+  uint8_t *out;
+  float *power;   // optional
+  float *bpart;   // [nbeam][nsamp/64][ngroup] per-tile beam power sums
+  int nant, nchan, ngroup, nbeam, nbeam_pad, nbt, wmode, order;
+  int ntblk, nbsets, ntiles, tiles_per_xcd;
+  uint32_t pstride, astride, gstride;     // input bytes per packet, antenna, channel group
+  uint32_t in_bytes, wfrag_bytes, wscale_bytes;
+};
+
+// Tile L -> (time block, group, beam set); time blocks are the fastest index
+// so the tiles in flight on one XCD share a channel group's weights in L2.
+struct TileId {
+  int tblk, g, bset;
+};
+__device__ __forceinline__ TileId tile_id(const BfArgs &a, int L) {
+  TileId t;
+  t.tblk = L % a.ntblk;
+  const int rest = L / a.ntblk;
+  t.g = rest % a.ngroup;
+  t.bset = rest / a.ngroup;
+  return t;
+}
+
+// Weight row of (channel, pol) cq of group g: fine mode has one per channel.
+__device__ __forceinline__ int weight_row(const BfArgs &a, int g, int cq) {
+  return (a.wmode ? g * 8 + (cq >> 1) : g) * 2 + (cq & 1);
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *p, uint32_t bytes) {
+  // raw (stride 0) buffer, gfx9 dword-3 format bits; reads past `bytes`
+  // return zero instead of touching memory
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+// Weight fragments of one (channel, pol) for the wave's BTW beam tiles and the
+// per-beam power factor that undoes the weight (and FP8 voltage) scaling.
+// load(): flat loads; load_buf(): buffer loads whose offsets beyond the lane
+// part are wave-uniform (SGPRs), so no per-load VGPR address is needed.
+template <int PREC> struct WFrag;
+template <> struct WFrag<0> {
+  h8v w[BTW][8];
+  float sc[BTW];
+  __device__ __forceinline__ void load(const uint8_t *wfrag, const float *wscale, long wrow_bt,
+                                       long sc_off, int lane, int n) {
+#pragma unroll
+    for (int bt = 0; bt < BTW; bt++)
+#pragma unroll
+      for (int s = 0; s < 8; s++)
+        w[bt][s] = *(const h8v *)(wfrag + (((wrow_bt + bt) * 8 + s) * 64 + lane) * 16);
+#pragma unroll
+    for (int bt = 0; bt < BTW; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
+  }
+  // wrow_bt = weight row * nbt + first beam tile, sc_off = weight row *
+  // nbeam_pad + first beam (both wave-uniform)
+  __device__ __forceinline__ void load_buf(__amdgpu_buffer_rsrc_t rw, __amdgpu_buffer_rsrc_t rs,
+                                           uint32_t wrow_bt, uint32_t sc_off, uint32_t vlane16,
+                                           uint32_t vn4) {
+    // the scales first: a register copy of one then waits only for it
+#pragma unroll
+    for (int bt = 0; bt < BTW; bt++)
+      sc[bt] = __builtin_bit_cast(
+          float, __builtin_amdgcn_raw_buffer_load_b32(rs, vn4, (sc_off + bt * 32) * 4u, 0));
+#pragma unroll
+    for (int bt = 0; bt < BTW; bt++)
+#pragma unroll
+      for (int s = 0; s < 8; s++)
+        w[bt][s] = __builtin_bit_cast(
+            h8v, __builtin_amdgcn_raw_buffer_load_b128(rw, vlane16, ((wrow_bt + bt) * 8 + s) * 1024u, 0));
+  }
+};
+template <> struct WFrag<1> {
+  i8v hi[BTW][2], lo[BTW][2];
+  float sc[BTW];
+  __device__ __forceinline__ void load(const uint8_t *wfrag, const float *wscale, long wrow_bt,
+                                       long sc_off, int lane, int n) {
+#pragma unroll
+    for (int bt = 0; bt < BTW; bt++)
+#pragma unroll
+      for (int s2 = 0; s2 < 2; s2++) {
+        const long rec = (((wrow_bt + bt) * 2 + s2) * 2) * 64 + lane;
+        hi[bt][s2] = *(const i8v *)(wfrag + rec * 32);
+        lo[bt][s2] = *(const i8v *)(wfrag + (rec + 64) * 32);
+      }
+#pragma unroll
+    for (int bt = 0; bt < BTW; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
+  }
+  __device__ __forceinline__ void load_buf(__amdgpu_buffer_rsrc_t rw, __amdgpu_buffer_rsrc_t rs,
+                                           uint32_t wrow_bt, uint32_t sc_off, uint32_t vlane16,
+                                           uint32_t vn4) {
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int bt = 0; bt < BTW; bt++)
+      sc[bt] = __builtin_bit_cast(
+          float, __builtin_amdgcn_raw_buffer_load_b32(rs, vn4, (sc_off + bt * 32) * 4u, 0));
+#pragma unroll
+    for (int bt = 0; bt < BTW; bt++)
+#pragma unroll
+      for (int s2 = 0; s2 < 2; s2++) {
+        // 32-byte records: lane l at rec*32 = base + 2*(16*l) -> two b128
+        // loads at 32*l and 32*l + 16 (voffset 2*vlane16)
+        const uint32_t base = (((wrow_bt + bt) * 2 + s2) * 2) * 64u * 32u;
+#pragma unroll
+        for (int hl = 0; hl < 2; hl++) {
+          const uint32_t so = base + hl * 64u * 32u;
+          const u4 a = __builtin_amdgcn_raw_buffer_load_b128(rw, 2 * vlane16, so, 0);
+          const u4 b = __builtin_amdgcn_raw_buffer_load_b128(rw, 2 * vlane16 + 16, so, 0);
+          const i8v v = {(int)a.x, (int)a.y, (int)a.z, (int)a.w, (int)b.x, (int)b.y, (int)b.z, (int)b.w};
+          if (hl == 0) hi[bt][s2] = v;
+          else lo[bt][s2] = v;
+        }
+      }
+  }
+};
+
+// One GEMM unit: 32 staged sample rows of one (channel, pol) against the
+// wave's weights.  Lane (n, h) reads row n, antennas 32h..32h+31.
+struct Acc {
+  f16v r[BTW], i[BTW];
+};
+
+template <int PREC>
+__device__ __forceinline__ void gemm_unit(Acc &A, const WFrag<PREC> &W, const uint8_t *rows, int n,
+                                          int h) {
+  const uint8_t *src = rows + n * ROWB + h * 32;
+  const uint4 d0 = *(const uint4 *)src, d1 = *(const uint4 *)(src + 16);
+  const uint32_t dw[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+#pragma unroll
+  for (int bt = 0; bt < BTW; bt++) { A.r[bt] = (f16v){}; A.i[bt] = (f16v){}; }
+  if constexpr (PREC == 0) {
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      h8v are, aim;
+      unpack_f16(dw[s], are, aim);
+#pragma unroll
+      for (int bt = 0; bt < BTW; bt++) {
+        A.r[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(are, W.w[bt][s], A.r[bt], 0, 0, 0);
+        A.i[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aim, W.w[bt][s], A.i[bt], 0, 0, 0);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int s2 = 0; s2 < 2; s2++) {
+      i8v are, aim;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const uint32_t x = dw[4 * s2 + u];
+        const uint32_t dr = nib_e4m3(x), di = nib_e4m3(x >> 4);
+        are[2 * u] = (int)dr;
+        are[2 * u + 1] = (int)(di ^ 0x80808080u);
+        aim[2 * u] = (int)di;
+        aim[2 * u + 1] = (int)dr;
+      }
+#pragma unroll
+      for (int bt = 0; bt < BTW; bt++) {
+        A.r[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.hi[bt][s2], A.r[bt], 0, 0, 0, 0, 0, 0);
+        A.i[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.hi[bt][s2], A.i[bt], 0, 0, 0, 0, 0, 0);
+        A.r[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.lo[bt][s2], A.r[bt], 0, 0, 0, 0, 0, 0);
+        A.i[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.lo[bt][s2], A.i[bt], 0, 0, 0, 0, 0, 0);
+      }
+    }
+  }
+}
+
+// |B|^2 summed over the accumulator rows of each bin (four independent FMA
+// chains per bin), scaled into the partial power.  Lane (n, h) of a 32x32
+// accumulator holds beam column n, rows 8*(v/4) + 4h + v%4.
+template <int TAVG>
+__device__ __forceinline__ void detect(const Acc &A, const float (&sc)[BTW],
+                                       float (&part)[BTW][TAVG <= 32 ? 32 / TAVG : 1]) {
+  constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;
+  constexpr int VPB = 16 / NP;
+#pragma unroll
+  for (int bt = 0; bt < BTW; bt++)
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+      float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+#pragma unroll
+      for (int v = j * VPB; v < (j + 1) * VPB; v += 2) {
+        s0 = __builtin_fmaf(A.r[bt][v], A.r[bt][v], s0);
+        s1 = __builtin_fmaf(A.i[bt][v], A.i[bt][v], s1);
+        s2 = __builtin_fmaf(A.r[bt][v + 1], A.r[bt][v + 1], s2);
+        s3 = __builtin_fmaf(A.i[bt][v + 1], A.i[bt][v + 1], s3);
+      }
+      part[bt][j] = __builtin_fmaf((s0 + s1) + (s2 + s3), sc[bt], part[bt][j]);
+    }
+}
+
+// 4x4 byte transposes: the 32-byte chunks of 4 antennas (aq*4..aq*4+3) of
+// packet pl -> one dword (4 antennas) in each of the 32 staged rows
+// (8 channels x 2 pols x 2 samples) they feed.
+__device__ __forceinline__ void stage_store(uint8_t *stage, const uint32_t (&v)[4][8], int pl, int aq) {
+  uint8_t *base = stage + aq * 4;
+#pragma unroll
+  for (int ci = 0; ci < 8; ci++) {
+    const uint32_t x0 = v[0][ci], x1 = v[1][ci], x2 = v[2][ci], x3 = v[3][ci];
+    const uint32_t l01 = __builtin_amdgcn_perm(x1, x0, 0x05010400u);
+    const uint32_t h01 = __builtin_amdgcn_perm(x1, x0, 0x07030602u);
+    const uint32_t l23 = __builtin_amdgcn_perm(x3, x2, 0x05010400u);
+    const uint32_t h23 = __builtin_amdgcn_perm(x3, x2, 0x07030602u);
+    *(uint32_t *)(base + ((ci * 2 + 0) * TCU + 2 * pl) * ROWB) = __builtin_amdgcn_perm(l23, l01, 0x05040100u);
+    *(uint32_t *)(base + ((ci * 2 + 1) * TCU + 2 * pl) * ROWB) = __builtin_amdgcn_perm(l23, l01, 0x07060302u);
+    *(uint32_t *)(base + ((ci * 2 + 0) * TCU + 2 * pl + 1) * ROWB) = __builtin_amdgcn_perm(h23, h01, 0x05040100u);
+    *(uint32_t *)(base + ((ci * 2 + 1) * TCU + 2 * pl + 1) * ROWB) = __builtin_amdgcn_perm(h23, h01, 0x07060302u);
+  }
+}
+
+// The tile's outputs: lanes h = 0 and h = 1 hold the two row halves of each
+// bin; the tile's sum per beam (the AGC input) goes to bpart.
+template <int TAVG>
+__device__ __forceinline__ void tile_output(const BfArgs &a, const TileId &t, int btw, int n, int h,
+                                            float (&part)[TAVG == 64 ? 1 : 2][BTW]
+                                                         [TAVG <= 32 ? 32 / TAVG : 1]) {
+  constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;
+  constexpr int NTT = TAVG == 64 ? 1 : 2;
+#pragma unroll
+  for (int bt = 0; bt < BTW; bt++) {
+    const int b = (btw + bt) * 32 + n;
+    float tile_sum = 0.f;
+#pragma unroll
+    for (int tt = 0; tt < NTT; tt++)
+#pragma unroll
+      for (int j = 0; j < NP; j++) {
+        float x = part[tt][bt][j];
+        x += __shfl_xor(x, 32, 64);
+        tile_sum += x;
+        if (h == 0 && b < a.nbeam) {
+          const int o = TAVG == 64 ? t.tblk : t.tblk * (TCU / TAVG) + tt * (32 / TAVG) + j;
+          const long idx = ((long)o * a.nbeam + b) * a.ngroup + t.g;
+          if (a.power) a.power[idx] = x;
+          a.out[idx] = requant_u8(x, a.scale[b]);
+        }
+      }
+    if (h == 0 && b < a.nbeam) a.bpart[((long)b * a.ntblk + t.tblk) * a.ngroup + t.g] = tile_sum;
+  }
+}
+
+// Host side: fill the tile arguments shared by the fused schedules.
+BfArgs bf_make_args(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
+                    const float *d_wscale, const float *d_scale, uint8_t *d_out, float *d_power,
+                    float *d_bpart, int bset_beams);
+
 #endif

@@ -1,6 +1,6 @@
 /*
  * bf_launch.h -- internal C ABI between the C host layer (host/bf_plan.c)
- * and the HIP kernels (csrc/bf_kernels.hip).  Not part of the public API.
+ * and the HIP kernels (csrc/ *.hip).  Not part of the public API.
  */
 #ifndef BF_LAUNCH_H
 #define BF_LAUNCH_H
@@ -31,7 +31,14 @@ typedef struct bfk_dims {
   int prec;       /* 0 FP16, 1 FP8 hi+lo                                  */
   int order;      /* 0 capture, 1 split -r                                */
   int stuff;      /* 0 none, 1 antennas, 2 incoherent beam 0              */
+  int sched;      /* fused-kernel schedule, BFK_SCHED_*                   */
 } bfk_dims;
+
+/* Schedules of the fused beamformer (same results, different speed):
+ * CLASSIC   one tile per workgroup, two workgroups per CU;
+ * PIPELINED one persistent workgroup per CU, staging and weight loads
+ *           overlapped with the MFMA work in software. */
+enum { BFK_SCHED_CLASSIC = 0, BFK_SCHED_PIPELINED = 1, BFK_NSCHED = 2 };
 
 /* Bytes of the weight fragment buffer and of the per-(channel,pol,beam)
  * weight scale buffer for dims. */

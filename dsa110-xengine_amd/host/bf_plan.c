@@ -240,3 +240,62 @@ int dsaxbf_sync(void *stream) {
   HIPCHK(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
   return DSAXBF_OK;
 }
+
+int dsaxbf_set_schedule(dsaxbf_plan *p, int sched) {
+  if (!p) return dsaxbf_fail(DSAXBF_EINVAL, "NULL plan");
+  if (sched < 0 || sched >= DSAXBF_NSCHED)
+    return dsaxbf_fail(DSAXBF_EINVAL, "schedule %d out of range", sched);
+  p->dims.sched = sched;
+  return DSAXBF_OK;
+}
+
+int dsaxbf_get_schedule(const dsaxbf_plan *p) { return p ? p->dims.sched : -1; }
+
+int dsaxbf_autotune(dsaxbf_plan *p, int reps, float *ms) {
+  dsaxbf_clear_error();
+  if (!p) return dsaxbf_fail(DSAXBF_EINVAL, "NULL plan");
+  if (reps < 1) return dsaxbf_fail(DSAXBF_EINVAL, "reps must be >= 1");
+  if (!p->have_weights && p->dims.stuff != DSAXBF_STUFF_ANTS)
+    return dsaxbf_fail(DSAXBF_ESTATE, "autotune before dsaxbf_set_weights");
+  HIPCHK(hipSetDevice(p->cfg.device), "hipSetDevice");
+  const size_t nin = dsaxbf_input_bytes(&p->cfg);
+  /* any voltages do: the work does not depend on the values */
+  HIPCHK(hipMemsetAsync(p->d_in, 0x5a, nin, p->stream), "hipMemsetAsync");
+  HIPCHK(hipMemsetAsync(p->d_scale, 0, sizeof(float) * p->cfg.nbeam, p->stream), "hipMemsetAsync");
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0), "hipEventCreate");
+  hipError_t he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    hipEventDestroy(e0);
+    return dsaxbf_fail(DSAXBF_EHIP, "hipEventCreate: %s", hipGetErrorString(he));
+  }
+  const int keep = p->dims.sched;
+  int best = keep, rc = DSAXBF_OK;
+  float best_ms = 0.f;
+  for (int s = 0; s < DSAXBF_NSCHED && rc == DSAXBF_OK; s++) {
+    p->dims.sched = s;
+    float t = 0.f;
+    for (int r = -1; r < reps && rc == DSAXBF_OK; r++) {
+      if (r == 0) he = hipEventRecord(e0, p->stream);
+      rc = dsaxbf_execute(p, p->d_in, p->d_scale, p->d_out, NULL, NULL, p->stream);
+    }
+    if (rc) break;
+    he = hipEventRecord(e1, p->stream);
+    if (he == hipSuccess) he = hipEventSynchronize(e1);
+    if (he == hipSuccess) he = hipEventElapsedTime(&t, e0, e1);
+    if (he != hipSuccess) {
+      rc = dsaxbf_fail(DSAXBF_EHIP, "autotune timing: %s", hipGetErrorString(he));
+      break;
+    }
+    t /= (float)reps;
+    if (ms) ms[s] = t;
+    if (s == 0 || t < best_ms) {
+      best_ms = t;
+      best = s;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  p->dims.sched = rc == DSAXBF_OK ? best : keep;
+  return rc;
+}

@@ -32,6 +32,10 @@ STUFF_ANTS = 1
 STUFF_IB = 2
 ORDER_CAPTURE = 0
 ORDER_SPLIT_R = 1
+SCHED_CLASSIC = 0
+SCHED_PIPELINED = 1
+NSCHED = 2
+SCHED_NAMES = {SCHED_CLASSIC: "classic", SCHED_PIPELINED: "pipelined"}
 
 ERRORS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "EIO", -5: "ESTATE", -6: "ENODEV"}
 
@@ -91,6 +95,9 @@ def _load():
         "dsaxbf_execute": (C.c_int, [P, P, P, P, P, P, P]),
         "dsaxbf_process_block": (C.c_int, [P, P, P, P, P, P]),
         "dsaxbf_beam_power": (C.c_int, [P, P, P]),
+        "dsaxbf_set_schedule": (C.c_int, [P, C.c_int]),
+        "dsaxbf_get_schedule": (C.c_int, [P]),
+        "dsaxbf_autotune": (C.c_int, [P, C.c_int, P]),
         "dsaxbf_agc_init": (C.c_int, [C.POINTER(AGC), C.c_int, C.c_int, C.c_int, C.c_int]),
         "dsaxbf_agc_free": (None, [C.POINTER(AGC)]),
         "dsaxbf_agc_update": (C.c_int, [C.POINTER(AGC), P, P, P]),
@@ -121,7 +128,8 @@ EXPORTED = [
     "dsaxbf_config_default", "dsaxbf_input_bytes", "dsaxbf_output_bytes", "dsaxbf_last_error",
     "dsaxbf_create", "dsaxbf_destroy", "dsaxbf_get_config", "dsaxbf_get_stream",
     "dsaxbf_read_weights_file", "dsaxbf_set_weights", "dsaxbf_reference_freqs",
-    "dsaxbf_execute", "dsaxbf_beam_power", "dsaxbf_process_block", "dsaxbf_agc_init", "dsaxbf_agc_free",
+    "dsaxbf_execute", "dsaxbf_beam_power", "dsaxbf_set_schedule", "dsaxbf_get_schedule",
+    "dsaxbf_autotune", "dsaxbf_process_block", "dsaxbf_agc_init", "dsaxbf_agc_free",
     "dsaxbf_agc_update", "dsaxbf_requant", "dsaxbf_synth_block", "dsaxbf_synth_geometry",
     "dsaxbf_dev_alloc", "dsaxbf_dev_free", "dsaxbf_copy_h2d", "dsaxbf_copy_d2h", "dsaxbf_sync",
     "dsaxbf_antpol_rms", "dsaxbf_version",
@@ -274,6 +282,21 @@ class Plan:
         """Per-beam power sum of the last execute (device buffer of nbeam floats)."""
         p = d_beam_power.ptr if isinstance(d_beam_power, DeviceBuffer) else C.c_void_p(d_beam_power)
         _check(lib.dsaxbf_beam_power(self._h, p, stream))
+
+    @property
+    def schedule(self) -> int:
+        return int(lib.dsaxbf_get_schedule(self._h))
+
+    @schedule.setter
+    def schedule(self, sched: int):
+        _check(lib.dsaxbf_set_schedule(self._h, sched))
+
+    def autotune(self, reps: int = 5) -> np.ndarray:
+        """Time every schedule on this plan's shape, keep the fastest; returns
+        the mean launch time (ms) of each."""
+        ms = np.zeros(NSCHED, np.float32)
+        _check(lib.dsaxbf_autotune(self._h, reps, _ptr(ms)))
+        return ms
 
     def close(self):
         if self._h:

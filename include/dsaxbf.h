@@ -176,6 +176,26 @@ int dsaxbf_execute(dsaxbf_plan *plan, const uint8_t *d_in, const float *d_scale,
  * itself when given d_beam_power.  Same stream as that execute. */
 int dsaxbf_beam_power(dsaxbf_plan *plan, float *d_beam_power, void *stream);
 
+/* Schedules of the fused kernel.  All give bit-identical results; which is
+ * fastest depends on the shape and the GPU.
+ *   CLASSIC    one tile per workgroup, two workgroups per CU;
+ *   PIPELINED  one persistent workgroup per CU with the next tile's staging
+ *              and the next weights' loads overlapped with the MFMA work.
+ * No counterpart in the reference (its five kernels per int have one fixed
+ * launch shape, dsaX_beamformer.cu:955-1003). */
+enum { DSAXBF_SCHED_CLASSIC = 0, DSAXBF_SCHED_PIPELINED = 1, DSAXBF_NSCHED = 2 };
+
+/* Select the schedule used by later executes on this plan (default CLASSIC). */
+int dsaxbf_set_schedule(dsaxbf_plan *plan, int sched);
+int dsaxbf_get_schedule(const dsaxbf_plan *plan);
+
+/* Autotuner: times every schedule on this plan's shape (the plan's own
+ * device buffers, current weights; reps launches each after one warm-up) and
+ * keeps the fastest.  ms, if not NULL, receives DSAXBF_NSCHED mean launch
+ * times in milliseconds.  Synchronous; requires dsaxbf_set_weights (or -s
+ * mode).  No counterpart in the reference. */
+int dsaxbf_autotune(dsaxbf_plan *plan, int reps, float *ms);
+
 /* Same as dsaxbf_execute with host pointers: copies the block in, runs it,
  * copies the output (and optional power / beam_power) back, then waits.
  * h_scale is [nbeam] floats on the host.  This is the psrdada-facing call

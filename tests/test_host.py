@@ -6,6 +6,7 @@ import ctypes as C
 import os
 import re
 import struct
+import shutil
 import subprocess
 
 import numpy as np
@@ -30,11 +31,14 @@ def test_every_declared_symbol_is_exported(bf):
     assert sorted(bf.EXPORTED) == names
 
 
-def test_library_is_gfx950_build(bf):
+def test_library_is_gfx950_build(bf, tmp_path):
     assert bf.lib.dsaxbf_version() == b"dsaxbf 1.0 gfx950"
-    so = bf.LIB_PATH
-    out = subprocess.run(["/opt/rocm/llvm/bin/llvm-objdump", "--offloading", so],
-                         capture_output=True, text=True)
+    # llvm-objdump --offloading writes the extracted code objects next to its
+    # input, so it reads a copy in a scratch folder
+    so = tmp_path / "lib.so"
+    shutil.copy(bf.LIB_PATH, so)
+    out = subprocess.run(["/opt/rocm/llvm/bin/llvm-objdump", "--offloading", str(so)],
+                         capture_output=True, text=True, cwd=tmp_path)
     if out.returncode == 0 and out.stdout:
         assert "gfx950" in out.stdout
 
