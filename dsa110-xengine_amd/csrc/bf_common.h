@@ -139,13 +139,27 @@ struct BfArgs {
   uint32_t in_bytes, wfrag_bytes, wscale_bytes;
 };
 
-// Tile L -> (time block, group, beam set); time blocks are the fastest index
-// so the tiles in flight on one XCD share a channel group's weights in L2.
+// Tile L -> (time block, group, beam set).  Consecutive tiles run at the same
+// time on one XCD.  When ngroup is a multiple of 4 the order is (group mod 4)
+// fastest, then time block, then group / 4: the four tiles that read the four
+// 32-byte quarters of each 128-byte input line run together, so one XCD's L2
+// fetches each line once, and the following tiles still share those groups'
+// weights in L2.  Otherwise time blocks are the fastest index.
 struct TileId {
   int tblk, g, bset;
 };
 __device__ __forceinline__ TileId tile_id(const BfArgs &a, int L) {
   TileId t;
+  if ((a.ngroup & 3) == 0) {
+    const int gl = L & 3;
+    int rest = L >> 2;
+    t.tblk = rest % a.ntblk;
+    rest /= a.ntblk;
+    const int gq = a.ngroup >> 2;
+    t.g = (rest % gq) * 4 + gl;
+    t.bset = rest / gq;
+    return t;
+  }
   t.tblk = L % a.ntblk;
   const int rest = L / a.ntblk;
   t.g = rest % a.ngroup;
