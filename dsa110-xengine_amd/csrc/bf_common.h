@@ -178,130 +178,147 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *p, uin
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
 }
 
-// Weight fragments of one (channel, pol) for the wave's BTW beam tiles and the
-// per-beam power factor that undoes the weight (and FP8 voltage) scaling.
+// Weight fragments of one (channel, pol) for the wave's NB beam tiles and NK
+// 64-antenna chunks, and the per-beam power factor that undoes the weight (and
+// FP8 voltage) scaling.  wrow_bt = weight row * nbt + first beam tile and
+// sc_off = weight row * nbeam_pad + first beam are wave-uniform.
 // load(): flat loads; load_buf(): buffer loads whose offsets beyond the lane
 // part are wave-uniform (SGPRs), so no per-load VGPR address is needed.
-template <int PREC> struct WFrag;
-template <> struct WFrag<0> {
-  h8v w[BTW][8];
-  float sc[BTW];
+template <int PREC, int NB = BTW, int NK = 1> struct WFrag;
+template <int NB, int NK> struct WFrag<0, NB, NK> {
+  h8v w[NB][NK][8];
+  float sc[NB];
   __device__ __forceinline__ void load(const uint8_t *wfrag, const float *wscale, long wrow_bt,
                                        long sc_off, int lane, int n) {
 #pragma unroll
-    for (int bt = 0; bt < BTW; bt++)
+    for (int bt = 0; bt < NB; bt++)
 #pragma unroll
-      for (int s = 0; s < 8; s++)
-        w[bt][s] = *(const h8v *)(wfrag + (((wrow_bt + bt) * 8 + s) * 64 + lane) * 16);
+      for (int kc = 0; kc < NK; kc++)
 #pragma unroll
-    for (int bt = 0; bt < BTW; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
+        for (int s = 0; s < 8; s++)
+          w[bt][kc][s] = *(const h8v *)(wfrag + ((((wrow_bt + bt) * NK + kc) * 8 + s) * 64 + lane) * 16);
+#pragma unroll
+    for (int bt = 0; bt < NB; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
   }
-  // wrow_bt = weight row * nbt + first beam tile, sc_off = weight row *
-  // nbeam_pad + first beam (both wave-uniform)
   __device__ __forceinline__ void load_buf(__amdgpu_buffer_rsrc_t rw, __amdgpu_buffer_rsrc_t rs,
                                            uint32_t wrow_bt, uint32_t sc_off, uint32_t vlane16,
                                            uint32_t vn4) {
     // the scales first: a register copy of one then waits only for it
 #pragma unroll
-    for (int bt = 0; bt < BTW; bt++)
+    for (int bt = 0; bt < NB; bt++)
       sc[bt] = __builtin_bit_cast(
           float, __builtin_amdgcn_raw_buffer_load_b32(rs, vn4, (sc_off + bt * 32) * 4u, 0));
 #pragma unroll
-    for (int bt = 0; bt < BTW; bt++)
+    for (int bt = 0; bt < NB; bt++)
 #pragma unroll
-      for (int s = 0; s < 8; s++)
-        w[bt][s] = __builtin_bit_cast(
-            h8v, __builtin_amdgcn_raw_buffer_load_b128(rw, vlane16, ((wrow_bt + bt) * 8 + s) * 1024u, 0));
+      for (int kc = 0; kc < NK; kc++)
+#pragma unroll
+        for (int s = 0; s < 8; s++)
+          w[bt][kc][s] = __builtin_bit_cast(
+              h8v, __builtin_amdgcn_raw_buffer_load_b128(
+                       rw, vlane16, (((wrow_bt + bt) * NK + kc) * 8 + s) * 1024u, 0));
   }
 };
-template <> struct WFrag<1> {
-  i8v hi[BTW][2], lo[BTW][2];
-  float sc[BTW];
+template <int NB, int NK> struct WFrag<1, NB, NK> {
+  i8v hi[NB][NK][2], lo[NB][NK][2];
+  float sc[NB];
   __device__ __forceinline__ void load(const uint8_t *wfrag, const float *wscale, long wrow_bt,
                                        long sc_off, int lane, int n) {
 #pragma unroll
-    for (int bt = 0; bt < BTW; bt++)
+    for (int bt = 0; bt < NB; bt++)
 #pragma unroll
-      for (int s2 = 0; s2 < 2; s2++) {
-        const long rec = (((wrow_bt + bt) * 2 + s2) * 2) * 64 + lane;
-        hi[bt][s2] = *(const i8v *)(wfrag + rec * 32);
-        lo[bt][s2] = *(const i8v *)(wfrag + (rec + 64) * 32);
-      }
+      for (int kc = 0; kc < NK; kc++)
 #pragma unroll
-    for (int bt = 0; bt < BTW; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
+        for (int s2 = 0; s2 < 2; s2++) {
+          const long rec = ((((wrow_bt + bt) * NK + kc) * 2 + s2) * 2) * 64 + lane;
+          hi[bt][kc][s2] = *(const i8v *)(wfrag + rec * 32);
+          lo[bt][kc][s2] = *(const i8v *)(wfrag + (rec + 64) * 32);
+        }
+#pragma unroll
+    for (int bt = 0; bt < NB; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
   }
   __device__ __forceinline__ void load_buf(__amdgpu_buffer_rsrc_t rw, __amdgpu_buffer_rsrc_t rs,
                                            uint32_t wrow_bt, uint32_t sc_off, uint32_t vlane16,
                                            uint32_t vn4) {
     typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 #pragma unroll
-    for (int bt = 0; bt < BTW; bt++)
+    for (int bt = 0; bt < NB; bt++)
       sc[bt] = __builtin_bit_cast(
           float, __builtin_amdgcn_raw_buffer_load_b32(rs, vn4, (sc_off + bt * 32) * 4u, 0));
 #pragma unroll
-    for (int bt = 0; bt < BTW; bt++)
+    for (int bt = 0; bt < NB; bt++)
 #pragma unroll
-      for (int s2 = 0; s2 < 2; s2++) {
-        // 32-byte records: lane l at rec*32 = base + 2*(16*l) -> two b128
-        // loads at 32*l and 32*l + 16 (voffset 2*vlane16)
-        const uint32_t base = (((wrow_bt + bt) * 2 + s2) * 2) * 64u * 32u;
+      for (int kc = 0; kc < NK; kc++)
 #pragma unroll
-        for (int hl = 0; hl < 2; hl++) {
-          const uint32_t so = base + hl * 64u * 32u;
-          const u4 a = __builtin_amdgcn_raw_buffer_load_b128(rw, 2 * vlane16, so, 0);
-          const u4 b = __builtin_amdgcn_raw_buffer_load_b128(rw, 2 * vlane16 + 16, so, 0);
-          const i8v v = {(int)a.x, (int)a.y, (int)a.z, (int)a.w, (int)b.x, (int)b.y, (int)b.z, (int)b.w};
-          if (hl == 0) hi[bt][s2] = v;
-          else lo[bt][s2] = v;
+        for (int s2 = 0; s2 < 2; s2++) {
+          // 32-byte records: lane l at rec*32 = base + 2*(16*l) -> two b128
+          // loads at 32*l and 32*l + 16 (voffset 2*vlane16)
+          const uint32_t base = ((((wrow_bt + bt) * NK + kc) * 2 + s2) * 2) * 64u * 32u;
+#pragma unroll
+          for (int hl = 0; hl < 2; hl++) {
+            const uint32_t so = base + hl * 64u * 32u;
+            const u4 a = __builtin_amdgcn_raw_buffer_load_b128(rw, 2 * vlane16, so, 0);
+            const u4 b = __builtin_amdgcn_raw_buffer_load_b128(rw, 2 * vlane16 + 16, so, 0);
+            const i8v v = {(int)a.x, (int)a.y, (int)a.z, (int)a.w,
+                           (int)b.x, (int)b.y, (int)b.z, (int)b.w};
+            if (hl == 0) hi[bt][kc][s2] = v;
+            else lo[bt][kc][s2] = v;
+          }
         }
-      }
   }
 };
 
 // One GEMM unit: 32 staged sample rows of one (channel, pol) against the
-// wave's weights.  Lane (n, h) reads row n, antennas 32h..32h+31.
-struct Acc {
-  f16v r[BTW], i[BTW];
+// wave's weights.  Rows are rowb bytes apart; lane (n, h) reads row n,
+// antennas 64kc + 32h .. 64kc + 32h + 31 for each chunk kc.
+template <int NB = BTW> struct Acc {
+  f16v r[NB], i[NB];
 };
 
-template <int PREC>
-__device__ __forceinline__ void gemm_unit(Acc &A, const WFrag<PREC> &W, const uint8_t *rows, int n,
-                                          int h) {
-  const uint8_t *src = rows + n * ROWB + h * 32;
-  const uint4 d0 = *(const uint4 *)src, d1 = *(const uint4 *)(src + 16);
-  const uint32_t dw[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+template <int PREC, int NB, int NK>
+__device__ __forceinline__ void gemm_unit(Acc<NB> &A, const WFrag<PREC, NB, NK> &W, const uint8_t *rows,
+                                          int rowb, int n, int h) {
 #pragma unroll
-  for (int bt = 0; bt < BTW; bt++) { A.r[bt] = (f16v){}; A.i[bt] = (f16v){}; }
-  if constexpr (PREC == 0) {
+  for (int bt = 0; bt < NB; bt++) { A.r[bt] = (f16v){}; A.i[bt] = (f16v){}; }
 #pragma unroll
-    for (int s = 0; s < 8; s++) {
-      h8v are, aim;
-      unpack_f16(dw[s], are, aim);
+  for (int kc = 0; kc < NK; kc++) {
+    const uint8_t *src = rows + n * rowb + kc * 64 + h * 32;
+    const uint4 d0 = *(const uint4 *)src, d1 = *(const uint4 *)(src + 16);
+    const uint32_t dw[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+    if constexpr (PREC == 0) {
 #pragma unroll
-      for (int bt = 0; bt < BTW; bt++) {
-        A.r[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(are, W.w[bt][s], A.r[bt], 0, 0, 0);
-        A.i[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aim, W.w[bt][s], A.i[bt], 0, 0, 0);
+      for (int s = 0; s < 8; s++) {
+        h8v are, aim;
+        unpack_f16(dw[s], are, aim);
+#pragma unroll
+        for (int bt = 0; bt < NB; bt++) {
+          A.r[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(are, W.w[bt][kc][s], A.r[bt], 0, 0, 0);
+          A.i[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aim, W.w[bt][kc][s], A.i[bt], 0, 0, 0);
+        }
       }
-    }
-  } else {
+    } else {
 #pragma unroll
-    for (int s2 = 0; s2 < 2; s2++) {
-      i8v are, aim;
+      for (int s2 = 0; s2 < 2; s2++) {
+        i8v are, aim;
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const uint32_t x = dw[4 * s2 + u];
-        const uint32_t dr = nib_e4m3(x), di = nib_e4m3(x >> 4);
-        are[2 * u] = (int)dr;
-        are[2 * u + 1] = (int)(di ^ 0x80808080u);
-        aim[2 * u] = (int)di;
-        aim[2 * u + 1] = (int)dr;
-      }
+        for (int u = 0; u < 4; u++) {
+          const uint32_t x = dw[4 * s2 + u];
+          const uint32_t dr = nib_e4m3(x), di = nib_e4m3(x >> 4);
+          are[2 * u] = (int)dr;
+          are[2 * u + 1] = (int)(di ^ 0x80808080u);
+          aim[2 * u] = (int)di;
+          aim[2 * u + 1] = (int)dr;
+        }
 #pragma unroll
-      for (int bt = 0; bt < BTW; bt++) {
-        A.r[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.hi[bt][s2], A.r[bt], 0, 0, 0, 0, 0, 0);
-        A.i[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.hi[bt][s2], A.i[bt], 0, 0, 0, 0, 0, 0);
-        A.r[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.lo[bt][s2], A.r[bt], 0, 0, 0, 0, 0, 0);
-        A.i[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.lo[bt][s2], A.i[bt], 0, 0, 0, 0, 0, 0);
+        for (int bt = 0; bt < NB; bt++) {
+          A.r[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.hi[bt][kc][s2], A.r[bt], 0, 0, 0, 0, 0, 0);
+          A.i[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.hi[bt][kc][s2], A.i[bt], 0, 0, 0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int bt = 0; bt < NB; bt++) {
+          A.r[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.lo[bt][kc][s2], A.r[bt], 0, 0, 0, 0, 0, 0);
+          A.i[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.lo[bt][kc][s2], A.i[bt], 0, 0, 0, 0, 0, 0);
+        }
       }
     }
   }
@@ -310,13 +327,13 @@ __device__ __forceinline__ void gemm_unit(Acc &A, const WFrag<PREC> &W, const ui
 // |B|^2 summed over the accumulator rows of each bin (four independent FMA
 // chains per bin), scaled into the partial power.  Lane (n, h) of a 32x32
 // accumulator holds beam column n, rows 8*(v/4) + 4h + v%4.
-template <int TAVG>
-__device__ __forceinline__ void detect(const Acc &A, const float (&sc)[BTW],
-                                       float (&part)[BTW][TAVG <= 32 ? 32 / TAVG : 1]) {
+template <int TAVG, int NB>
+__device__ __forceinline__ void detect(const Acc<NB> &A, const float (&sc)[NB],
+                                       float (&part)[NB][TAVG <= 32 ? 32 / TAVG : 1]) {
   constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;
   constexpr int VPB = 16 / NP;
 #pragma unroll
-  for (int bt = 0; bt < BTW; bt++)
+  for (int bt = 0; bt < NB; bt++)
 #pragma unroll
     for (int j = 0; j < NP; j++) {
       float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
@@ -331,35 +348,37 @@ __device__ __forceinline__ void detect(const Acc &A, const float (&sc)[BTW],
     }
 }
 
-// 4x4 byte transposes: the 32-byte chunks of 4 antennas (aq*4..aq*4+3) of
-// packet pl -> one dword (4 antennas) in each of the 32 staged rows
-// (8 channels x 2 pols x 2 samples) they feed.
-__device__ __forceinline__ void stage_store(uint8_t *stage, const uint32_t (&v)[4][8], int pl, int aq) {
+// 4x4 byte transposes: the NCI dwords (each 1 channel x 2 times x 2 pols) of
+// 4 antennas (aq*4..aq*4+3) of packet pl -> one dword (4 antennas) in each of
+// the 4*NCI staged rows ([channel][pol][sample], rowb bytes apart) they feed.
+template <int NCI>
+__device__ __forceinline__ void stage_store(uint8_t *stage, const uint32_t (&v)[4][NCI], int pl, int aq,
+                                            int rowb) {
   uint8_t *base = stage + aq * 4;
 #pragma unroll
-  for (int ci = 0; ci < 8; ci++) {
+  for (int ci = 0; ci < NCI; ci++) {
     const uint32_t x0 = v[0][ci], x1 = v[1][ci], x2 = v[2][ci], x3 = v[3][ci];
     const uint32_t l01 = __builtin_amdgcn_perm(x1, x0, 0x05010400u);
     const uint32_t h01 = __builtin_amdgcn_perm(x1, x0, 0x07030602u);
     const uint32_t l23 = __builtin_amdgcn_perm(x3, x2, 0x05010400u);
     const uint32_t h23 = __builtin_amdgcn_perm(x3, x2, 0x07030602u);
-    *(uint32_t *)(base + ((ci * 2 + 0) * TCU + 2 * pl) * ROWB) = __builtin_amdgcn_perm(l23, l01, 0x05040100u);
-    *(uint32_t *)(base + ((ci * 2 + 1) * TCU + 2 * pl) * ROWB) = __builtin_amdgcn_perm(l23, l01, 0x07060302u);
-    *(uint32_t *)(base + ((ci * 2 + 0) * TCU + 2 * pl + 1) * ROWB) = __builtin_amdgcn_perm(h23, h01, 0x05040100u);
-    *(uint32_t *)(base + ((ci * 2 + 1) * TCU + 2 * pl + 1) * ROWB) = __builtin_amdgcn_perm(h23, h01, 0x07060302u);
+    *(uint32_t *)(base + ((ci * 2 + 0) * TCU + 2 * pl) * rowb) = __builtin_amdgcn_perm(l23, l01, 0x05040100u);
+    *(uint32_t *)(base + ((ci * 2 + 1) * TCU + 2 * pl) * rowb) = __builtin_amdgcn_perm(l23, l01, 0x07060302u);
+    *(uint32_t *)(base + ((ci * 2 + 0) * TCU + 2 * pl + 1) * rowb) = __builtin_amdgcn_perm(h23, h01, 0x05040100u);
+    *(uint32_t *)(base + ((ci * 2 + 1) * TCU + 2 * pl + 1) * rowb) = __builtin_amdgcn_perm(h23, h01, 0x07060302u);
   }
 }
 
 // The tile's outputs: lanes h = 0 and h = 1 hold the two row halves of each
 // bin; the tile's sum per beam (the AGC input) goes to bpart.
-template <int TAVG>
+template <int TAVG, int NB>
 __device__ __forceinline__ void tile_output(const BfArgs &a, const TileId &t, int btw, int n, int h,
-                                            float (&part)[TAVG == 64 ? 1 : 2][BTW]
+                                            float (&part)[TAVG == 64 ? 1 : 2][NB]
                                                          [TAVG <= 32 ? 32 / TAVG : 1]) {
   constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;
   constexpr int NTT = TAVG == 64 ? 1 : 2;
 #pragma unroll
-  for (int bt = 0; bt < BTW; bt++) {
+  for (int bt = 0; bt < NB; bt++) {
     const int b = (btw + bt) * 32 + n;
     float tile_sum = 0.f;
 #pragma unroll

@@ -82,7 +82,7 @@ __device__ __forceinline__ void raw_transpose(const uint8_t *raw, uint8_t *stage
       v[j][4] = live ? c1.x : 0u; v[j][5] = live ? c1.y : 0u;
       v[j][6] = live ? c1.z : 0u; v[j][7] = live ? c1.w : 0u;
     }
-    stage_store(stage, v, pl, aq);
+    stage_store(stage, v, pl, aq, ROWB);
   }
 }
 
@@ -163,14 +163,14 @@ void bf_pipelined_kernel(BfArgs a) {
     // scheduling region so the scheduler interleaves within it instead of
     // hoisting later units.
     WFrag<PREC> W0, W1;
-    Acc A0, A1;
+    Acc<> A0, A1;
     float sc_prev[BTW];
     W0.load_buf(rw, rs, wrow_bt(0), sc_off(0), vlane16, vn4);
     W1.load_buf(rw, rs, wrow_bt(1), sc_off(1), vlane16, vn4);
     uint4 slice = raw_fetch(rin, to_next, a, wave, voff_even, voff_odd, 0);
-    gemm_unit<PREC>(A0, W0, stage, n, h);                                       // unit 0
+    gemm_unit(A0, W0, stage, ROWB, n, h);                                       // unit 0
     __builtin_amdgcn_sched_barrier(0);
-    gemm_unit<PREC>(A1, W0, stage + 32 * ROWB, n, h);                           // unit 1
+    gemm_unit(A1, W0, stage + 32 * ROWB, ROWB, n, h);                           // unit 1
     detect<TAVG>(A0, W0.sc, part[0]);                                           // unit 0
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -181,10 +181,10 @@ void bf_pipelined_kernel(BfArgs a) {
       raw_put(raw, wave, lane, cq - 1, slice);
       W0.load_buf(rw, rs, wrow_bt(cq + 1), sc_off(cq + 1), vlane16, vn4);
       slice = raw_fetch(rin, to_next, a, wave, voff_even, voff_odd, cq);
-      gemm_unit<PREC>(A0, W1, stage + (cq * TCU) * ROWB, n, h);                // unit 2cq
+      gemm_unit(A0, W1, stage + (cq * TCU) * ROWB, ROWB, n, h);                // unit 2cq
       detect<TAVG>(A1, sc_prev, part[NTT - 1]);                                 // unit 2cq-1
       __builtin_amdgcn_sched_barrier(0);
-      gemm_unit<PREC>(A1, W1, stage + (cq * TCU + 32) * ROWB, n, h);           // unit 2cq+1
+      gemm_unit(A1, W1, stage + (cq * TCU + 32) * ROWB, ROWB, n, h);           // unit 2cq+1
       detect<TAVG>(A0, W1.sc, part[0]);                                          // unit 2cq
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -193,10 +193,10 @@ void bf_pipelined_kernel(BfArgs a) {
       raw_put(raw, wave, lane, cq, slice);
       W1.load_buf(rw, rs, wrow_bt(cq + 2), sc_off(cq + 2), vlane16, vn4);
       slice = raw_fetch(rin, to_next, a, wave, voff_even, voff_odd, cq + 1);
-      gemm_unit<PREC>(A0, W0, stage + ((cq + 1) * TCU) * ROWB, n, h);          // unit 2cq+2
+      gemm_unit(A0, W0, stage + ((cq + 1) * TCU) * ROWB, ROWB, n, h);          // unit 2cq+2
       detect<TAVG>(A1, sc_prev, part[NTT - 1]);                                 // unit 2cq+1
       __builtin_amdgcn_sched_barrier(0);
-      gemm_unit<PREC>(A1, W0, stage + ((cq + 1) * TCU + 32) * ROWB, n, h);     // unit 2cq+3
+      gemm_unit(A1, W0, stage + ((cq + 1) * TCU + 32) * ROWB, ROWB, n, h);     // unit 2cq+3
       detect<TAVG>(A0, W0.sc, part[0]);                                          // unit 2cq+2
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -205,10 +205,10 @@ void bf_pipelined_kernel(BfArgs a) {
     // ---- (channel, pol) 15 with W1
     raw_put(raw, wave, lane, 14, slice);
     slice = raw_fetch(rin, to_next, a, wave, voff_even, voff_odd, 15);
-    gemm_unit<PREC>(A0, W1, stage + (15 * TCU) * ROWB, n, h);                   // unit 30
+    gemm_unit(A0, W1, stage + (15 * TCU) * ROWB, ROWB, n, h);                   // unit 30
     detect<TAVG>(A1, sc_prev, part[NTT - 1]);                                   // unit 29
     __builtin_amdgcn_sched_barrier(0);
-    gemm_unit<PREC>(A1, W1, stage + (15 * TCU + 32) * ROWB, n, h);              // unit 31
+    gemm_unit(A1, W1, stage + (15 * TCU + 32) * ROWB, ROWB, n, h);              // unit 31
     detect<TAVG>(A0, W1.sc, part[0]);                                           // unit 30
     __builtin_amdgcn_sched_barrier(0);
     detect<TAVG>(A1, W1.sc, part[NTT - 1]);                                     // unit 31
@@ -264,6 +264,12 @@ BfArgs bf_make_args(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
 extern "C" int bfk_beamform_classic(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
                                     const float *d_wscale, const float *d_scale, uint8_t *d_out,
                                     float *d_power, float *d_bpart, void *stream);
+
+extern "C" int bfk_sched_supported(const bfk_dims *d, int sched) {
+  if (sched == BFK_SCHED_CLASSIC) return d->nkc == 1 || d->nkc == 2 || d->nkc == 4;
+  if (sched == BFK_SCHED_PIPELINED) return d->nkc == 1 && d->nbeam_pad % BSET == 0;
+  return 0;
+}
 
 static int beamform_pipelined(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
                               const float *d_wscale, const float *d_scale, uint8_t *d_out,

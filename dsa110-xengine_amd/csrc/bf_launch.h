@@ -60,6 +60,9 @@ int bfk_beamform(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
                  const float *d_wscale, const float *d_scale, uint8_t *d_out,
                  float *d_power, float *d_bpart, void *stream);
 
+/* 1 if schedule sched (BFK_SCHED_*) can run dims d, else 0. */
+int bfk_sched_supported(const bfk_dims *d, int sched);
+
 /* Antenna-stuffing (-s) and incoherent-beam (-q) outputs (rewrites the
  * d_bpart entries of the beams it writes). */
 int bfk_stuff(const bfk_dims *d, const uint8_t *d_in, const float *d_scale,

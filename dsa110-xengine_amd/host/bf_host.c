@@ -60,8 +60,8 @@ size_t dsaxbf_output_bytes(const dsaxbf_config *cfg) {
 
 int dsaxbf_check_config(const dsaxbf_config *c) {
   if (!c) return dsaxbf_fail(DSAXBF_EINVAL, "config is NULL");
-  if (c->nant < 1 || c->nant > 64)
-    return dsaxbf_fail(DSAXBF_EINVAL, "nant=%d outside 1..64", c->nant);
+  if (c->nant < 1 || c->nant > 256)
+    return dsaxbf_fail(DSAXBF_EINVAL, "nant=%d outside 1..256", c->nant);
   if (c->nchan < 8 || c->nchan % 8)
     return dsaxbf_fail(DSAXBF_EINVAL, "nchan=%d must be a positive multiple of 8", c->nchan);
   if (c->npkt < 32 || c->npkt % 32)

@@ -41,7 +41,7 @@ struct dsaxbf_plan {
 static void fill_dims(const dsaxbf_config *c, bfk_dims *d) {
   memset(d, 0, sizeof(*d));
   d->nant = c->nant;
-  d->nant_pad = (c->nant + 63) / 64 * 64;
+  d->nant_pad = c->nant <= 64 ? 64 : c->nant <= 128 ? 128 : 256;   /* 1, 2 or 4 chunks */
   d->nkc = d->nant_pad / 64;
   d->nchan = c->nchan;
   d->ngroup = c->nchan / 8;
@@ -245,6 +245,9 @@ int dsaxbf_set_schedule(dsaxbf_plan *p, int sched) {
   if (!p) return dsaxbf_fail(DSAXBF_EINVAL, "NULL plan");
   if (sched < 0 || sched >= DSAXBF_NSCHED)
     return dsaxbf_fail(DSAXBF_EINVAL, "schedule %d out of range", sched);
+  if (!bfk_sched_supported(&p->dims, sched))
+    return dsaxbf_fail(DSAXBF_EINVAL, "schedule %d cannot run this shape (nant %d, nbeam %d)", sched,
+                       p->cfg.nant, p->cfg.nbeam);
   p->dims.sched = sched;
   return DSAXBF_OK;
 }
@@ -271,8 +274,12 @@ int dsaxbf_autotune(dsaxbf_plan *p, int reps, float *ms) {
   }
   const int keep = p->dims.sched;
   int best = keep, rc = DSAXBF_OK;
-  float best_ms = 0.f;
+  float best_ms = -1.f;
   for (int s = 0; s < DSAXBF_NSCHED && rc == DSAXBF_OK; s++) {
+    if (!bfk_sched_supported(&p->dims, s)) {   /* not a candidate for this shape */
+      if (ms) ms[s] = -1.f;
+      continue;
+    }
     p->dims.sched = s;
     float t = 0.f;
     for (int r = -1; r < reps && rc == DSAXBF_OK; r++) {
@@ -289,7 +296,7 @@ int dsaxbf_autotune(dsaxbf_plan *p, int reps, float *ms) {
     }
     t /= (float)reps;
     if (ms) ms[s] = t;
-    if (s == 0 || t < best_ms) {
+    if (best_ms < 0.f || t < best_ms) {
       best_ms = t;
       best = s;
     }

@@ -185,14 +185,16 @@ int dsaxbf_beam_power(dsaxbf_plan *plan, float *d_beam_power, void *stream);
  * launch shape, dsaX_beamformer.cu:955-1003). */
 enum { DSAXBF_SCHED_CLASSIC = 0, DSAXBF_SCHED_PIPELINED = 1, DSAXBF_NSCHED = 2 };
 
-/* Select the schedule used by later executes on this plan (default CLASSIC). */
+/* Select the schedule used by later executes on this plan (default CLASSIC);
+ * DSAXBF_EINVAL if it cannot run the plan's shape. */
 int dsaxbf_set_schedule(dsaxbf_plan *plan, int sched);
 int dsaxbf_get_schedule(const dsaxbf_plan *plan);
 
 /* Autotuner: times every schedule on this plan's shape (the plan's own
  * device buffers, current weights; reps launches each after one warm-up) and
  * keeps the fastest.  ms, if not NULL, receives DSAXBF_NSCHED mean launch
- * times in milliseconds.  Synchronous; requires dsaxbf_set_weights (or -s
+ * times in milliseconds (-1 for a schedule that cannot run this shape: the
+ * pipelined one needs nant <= 64).  Synchronous; requires dsaxbf_set_weights (or -s
  * mode).  No counterpart in the reference. */
 int dsaxbf_autotune(dsaxbf_plan *plan, int reps, float *ms);
 

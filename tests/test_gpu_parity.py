@@ -85,6 +85,20 @@ def test_fused_matches_oracle_shapes(bf, orc, prec, wmode, amode, order, nant, n
                 nchan=24, seed=nant + nbeam), prec)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("nant,nbeam,order,wmode,tavg", [
+    (100, 256, 0, 1, 8),       # two 64-antenna chunks, padded
+    (128, 300, 1, 0, 32),      # two full chunks, split -r order, group weights
+    (200, 512, 0, 1, 16),      # four chunks, padded
+    (256, 1024, 0, 1, 32),     # the scaled array: 256 antennas, 1024 beams
+])
+def test_fused_many_antennas(bf, orc, prec, nant, nbeam, order, wmode, tavg):
+    """More than 64 antennas: K in 64-antenna chunks, channels staged in
+    sub-steps (one 32-beam tile per wave)."""
+    _check(_run(bf, orc, prec=prec, nant=nant, nbeam=nbeam, order=order, wmode=wmode, tavg=tavg,
+                nchan=16, npkt=128, seed=nant), prec)
+
+
 @pytest.mark.parametrize("stuff", [1, 2])
 @pytest.mark.parametrize("tavg", [8, 32])
 def test_stuff_modes_exact(bf, orc, stuff, tavg):

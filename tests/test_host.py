@@ -58,7 +58,7 @@ def test_config_defaults_match_reference(bf):
 
 
 @pytest.mark.parametrize("field,value", [
-    ("nant", 0), ("nant", 65), ("nchan", 12), ("npkt", 48), ("nbeam", 0), ("nbeam", 1025),
+    ("nant", 0), ("nant", 257), ("nchan", 12), ("npkt", 48), ("nbeam", 0), ("nbeam", 1025),
     ("tavg", 4), ("tavg", 24), ("precision", 2), ("weight_mode", 3), ("angle_mode", -1),
     ("stuffants", 3), ("input_order", 2)])
 def test_create_rejects_bad_config(bf, field, value):

@@ -180,7 +180,8 @@ def _numpy_power(block, W, nant, nchan, npkt, nbeam, tavg, wmode, order=0):
 
 
 @pytest.mark.parametrize("wmode,amode,tavg,order,nant", [
-    (1, 1, 8, 0, 64), (0, 0, 16, 0, 63), (1, 0, 32, 1, 30), (0, 1, 64, 1, 24)])
+    (1, 1, 8, 0, 64), (0, 0, 16, 0, 63), (1, 0, 32, 1, 30), (0, 1, 64, 1, 24), (1, 1, 16, 0, 256),
+    (0, 0, 32, 1, 130)])
 def test_power_matches_numpy(orc, bf, wmode, amode, tavg, order, nant):
     nchan, npkt, nbeam = 16, 64, 40
     cfg = bf.config(nant=nant, nchan=nchan, npkt=npkt, nbeam=nbeam, tavg=tavg, input_order=order)
