@@ -98,6 +98,11 @@ def _load():
         "dsaxbf_set_schedule": (C.c_int, [P, C.c_int]),
         "dsaxbf_get_schedule": (C.c_int, [P]),
         "dsaxbf_autotune": (C.c_int, [P, C.c_int, P]),
+        "dsaxbf_pipe_create": (C.c_int, [C.POINTER(P), P]),
+        "dsaxbf_pipe_destroy": (None, [P]),
+        "dsaxbf_pipe_input": (P, [P, C.c_int]),
+        "dsaxbf_pipe_upload": (C.c_int, [P, C.c_int]),
+        "dsaxbf_pipe_run": (C.c_int, [P, C.c_int, P, C.POINTER(P), C.POINTER(P)]),
         "dsaxbf_agc_init": (C.c_int, [C.POINTER(AGC), C.c_int, C.c_int, C.c_int, C.c_int]),
         "dsaxbf_agc_free": (None, [C.POINTER(AGC)]),
         "dsaxbf_agc_update": (C.c_int, [C.POINTER(AGC), P, P, P]),
@@ -129,7 +134,8 @@ EXPORTED = [
     "dsaxbf_create", "dsaxbf_destroy", "dsaxbf_get_config", "dsaxbf_get_stream",
     "dsaxbf_read_weights_file", "dsaxbf_set_weights", "dsaxbf_reference_freqs",
     "dsaxbf_execute", "dsaxbf_beam_power", "dsaxbf_set_schedule", "dsaxbf_get_schedule",
-    "dsaxbf_autotune", "dsaxbf_process_block", "dsaxbf_agc_init", "dsaxbf_agc_free",
+    "dsaxbf_autotune", "dsaxbf_pipe_create", "dsaxbf_pipe_destroy", "dsaxbf_pipe_input",
+    "dsaxbf_pipe_upload", "dsaxbf_pipe_run", "dsaxbf_process_block", "dsaxbf_agc_init", "dsaxbf_agc_free",
     "dsaxbf_agc_update", "dsaxbf_requant", "dsaxbf_synth_block", "dsaxbf_synth_geometry",
     "dsaxbf_dev_alloc", "dsaxbf_dev_free", "dsaxbf_copy_h2d", "dsaxbf_copy_d2h", "dsaxbf_sync",
     "dsaxbf_antpol_rms", "dsaxbf_version",
@@ -301,6 +307,46 @@ class Plan:
     def close(self):
         if self._h:
             lib.dsaxbf_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Pipe:
+    """Two-slot streaming pipe over a plan (pinned, double-buffered uploads)."""
+
+    def __init__(self, plan: "Plan"):
+        self.plan = plan
+        self._h = C.c_void_p()
+        _check(lib.dsaxbf_pipe_create(C.byref(self._h), plan._h))
+        cfg = plan.cfg
+        self._nin, self._nout = input_bytes(cfg), output_bytes(cfg)
+        self._shape = (cfg.nout, cfg.nbeam, cfg.ngroup)
+
+    def input(self, slot: int) -> np.ndarray:
+        """The slot's pinned host input buffer as a writable uint8 array."""
+        addr = lib.dsaxbf_pipe_input(self._h, slot)
+        return np.ctypeslib.as_array((C.c_uint8 * self._nin).from_address(addr))
+
+    def upload(self, slot: int):
+        _check(lib.dsaxbf_pipe_upload(self._h, slot))
+
+    def run(self, slot: int, scale: np.ndarray):
+        """Beamform the slot's uploaded block; returns copies of (out, beam_power)."""
+        sc = np.ascontiguousarray(scale, np.float32)
+        o, b = C.c_void_p(), C.c_void_p()
+        _check(lib.dsaxbf_pipe_run(self._h, slot, _ptr(sc), C.byref(o), C.byref(b)))
+        out = np.ctypeslib.as_array((C.c_uint8 * self._nout).from_address(o.value)).reshape(self._shape)
+        bp = np.ctypeslib.as_array((C.c_float * self.plan.cfg.nbeam).from_address(b.value))
+        return out.copy(), bp.copy()
+
+    def close(self):
+        if self._h:
+            lib.dsaxbf_pipe_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -256,6 +256,25 @@ int dsaxbf_synth_geometry(int nant, int ncal, int nbeam, uint64_t seed,
                           double half_span_deg, float *antpos, float *cal,
                           double *theta_rad);
 
+/* ---- streaming (replaces the per-int copy/launch loop on 4 streams with
+ *      pageable copies, dsaX_beamformer.cu:955-1003) -------------------- */
+/* A pipe streams blocks through a plan with two pinned host input slots and
+ * two device slots: a block's upload (copy stream) overlaps the previous
+ * block's beamforming (plan stream).  Usage per block k:
+ *   fill dsaxbf_pipe_input(pp, (k+1)%2); dsaxbf_pipe_upload(pp, (k+1)%2);
+ *   dsaxbf_pipe_run(pp, k%2, scale, &out, &bp);   (then AGC -> next scale)
+ * with slot 0 filled and uploaded before the first run.  A slot's input may be
+ * refilled once dsaxbf_pipe_run on that slot has returned.  run blocks until
+ * the block's outputs are in the pipe's pinned buffers, whose pointers it
+ * returns (valid until the next run on the same slot). */
+typedef struct dsaxbf_pipe dsaxbf_pipe;
+int dsaxbf_pipe_create(dsaxbf_pipe **pipe, dsaxbf_plan *plan);
+void dsaxbf_pipe_destroy(dsaxbf_pipe *pipe);
+uint8_t *dsaxbf_pipe_input(dsaxbf_pipe *pipe, int slot);  /* pinned, input_bytes */
+int dsaxbf_pipe_upload(dsaxbf_pipe *pipe, int slot);
+int dsaxbf_pipe_run(dsaxbf_pipe *pipe, int slot, const float *h_scale,
+                    const uint8_t **h_out, const float **h_beam_power);
+
 /* ---- device memory helpers (for bindings without a GPU array library) --- */
 int dsaxbf_dev_alloc(void **d_ptr, size_t bytes, int device);
 int dsaxbf_dev_free(void *d_ptr);

@@ -338,3 +338,37 @@ def test_cli_end_to_end(bf, orc, tmp_path):
         if k > 0:
             assert np.array_equal(outs[k], out.ravel()), f"block {k}"
         scale, _ = agc.update(bp)
+
+
+def test_pipe_streams_like_process_block(bf):
+    """Three blocks through the two-slot pipe (uploads overlapping runs, AGC
+    between blocks) give the same bytes and beam powers as process_block."""
+    cfg = bf.config(nant=48, nchan=32, npkt=256, nbeam=256, tavg=16, weight_mode=1, angle_mode=1)
+    antpos, cal, theta = bf.synth_geometry(48, 32, 256, 21)
+    freqs = bf.reference_freqs(1498.75, 32)
+    blocks = [bf.synth_block(cfg, 60 + k) for k in range(3)]
+    plan = bf.Plan(cfg)
+    plan.set_weights(antpos, cal, freqs, theta)
+    agc = bf.Agc(256, cfg.ngroup, cfg.npkt // 16)
+    scale = np.full(256, 0.01, np.float32)
+    want = []
+    for blk in blocks:
+        out, _, bp = plan.process_block(blk, scale, False, True)
+        want.append((out, bp))
+        scale, _ = agc.update(bp)
+    pipe = bf.Pipe(plan)
+    agc = bf.Agc(256, cfg.ngroup, cfg.npkt // 16)
+    scale = np.full(256, 0.01, np.float32)
+    pipe.input(0)[:] = blocks[0]
+    pipe.upload(0)
+    for k in range(3):
+        if k + 1 < 3:
+            pipe.input((k + 1) % 2)[:] = blocks[k + 1]
+            pipe.upload((k + 1) % 2)
+        out, bp = pipe.run(k % 2, scale)
+        assert np.array_equal(out, want[k][0]) and np.array_equal(bp, want[k][1]), k
+        scale, _ = agc.update(bp)
+    with pytest.raises(bf.BeamformerError):
+        pipe.run(0, scale)                     # slot 0 not uploaded again
+    pipe.close()
+    plan.close()
