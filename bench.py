@@ -84,16 +84,16 @@ def workload(dsaxbf, nblocks, seed=2024):
 
 
 def time_path(dsaxbf, ev, prec_name, cfg, antpos, cal, theta, freqs, blocks, steps, warmup,
-              schedule="auto"):
+              variant="auto"):
     cfg.precision = dsaxbf.PREC_FP16 if prec_name == "fp16" else dsaxbf.PREC_FP8SPLIT
     plan = dsaxbf.Plan(cfg)
     plan.set_weights(antpos, cal, freqs, theta)
     tune_ms = None
-    if schedule == "auto":      # the library's autotuner picks the schedule
+    if variant == "auto":       # the library's autotuner picks the kernel variant
         tune_ms = plan.autotune(5)
     else:
-        plan.schedule = {v: k for k, v in dsaxbf.SCHED_NAMES.items()}[schedule]
-    sched_name = dsaxbf.SCHED_NAMES[plan.schedule]
+        plan.variant = {v: k for k, v in dsaxbf.VARIANT_NAMES.items()}[variant]
+    var_name = dsaxbf.VARIANT_NAMES[plan.variant]
     stream = plan.stream
     nout = cfg.nout * cfg.nbeam * cfg.ngroup
     d_in = [dsaxbf.DeviceBuffer(b.nbytes) for b in blocks]
@@ -141,9 +141,9 @@ def time_path(dsaxbf, ev, prec_name, cfg, antpos, cal, theta, freqs, blocks, ste
         b.free()
     return dict(ms_per_step=total_ms / steps, kernel_ms=float(np.mean(kern_ms)),
                 kernel_ms_min=float(np.min(kern_ms)), out=out, power=pw, scale=scale,
-                schedule=sched_name,
+                variant=var_name,
                 autotune_ms=None if tune_ms is None else
-                {dsaxbf.SCHED_NAMES[i]: float(t) for i, t in enumerate(tune_ms)})
+                {dsaxbf.VARIANT_NAMES[i]: float(t) for i, t in enumerate(tune_ms) if t > 0})
 
 
 def errors(orc, cfg, block, W, res, groups):
@@ -250,7 +250,7 @@ def stream_bench(dsaxbf, ev, nblocks=8, seed=77):
     wbytes = nc * 2 * (nb // 32) * (na // 64) * 64 * 128
     return {
         "workload": "256 ant x 2 pol x 384 ch x 4096 samples (805 MB packed) -> 1024 beams over "
-                    "+-3 deg, fine-channel weights, tavg 32, FP8-split, classic schedule",
+                    "+-3 deg, fine-channel weights, tavg 32, FP8-split, classic 128-beam tiles",
         "blocks_streamed": nblocks, "distinct_blocks": 2, "synth_s": synth_s,
         "s_per_block": wall, "realtime_margin": 0.134217728 / wall,
         "h2d_ms": h2d_ms, "h2d_GB_s": nin / (h2d_ms / 1e3) / 1e9,
@@ -259,6 +259,81 @@ def stream_bench(dsaxbf, ev, nblocks=8, seed=77):
         "kernel_algorithmic_GB_s": (nin + wbytes) / (kern_ms / 1e3) / 1e9,
         "first_two_blocks_s": first_two_s,
     }
+
+
+def batch_bench(dsaxbf, ev, seed=2025, nblocks=24, reps=5):
+    """Uneven blocks in one graph launch: nblocks blocks of 64 antennas with
+    channel counts from {48, 96, 192, 384}, beam counts from {64, 128, 256, 512}
+    and time lengths from {1024, 2048, 4096} samples (seeded draws), 4-bit
+    Gaussian voltages, geometry-derived fine-channel weights, tavg 32; run on
+    both precisions.  Reported: the graph replay time of the batch with the
+    autotuned tile shapes, against the same blocks run one execute at a time
+    with the default shape."""
+    rng = np.random.default_rng(seed)
+    shapes = [(int(rng.choice([48, 96, 192, 384])), int(rng.choice([64, 128, 256, 512])),
+               int(rng.choice([1024, 2048, 4096]))) for _ in range(nblocks)]
+    res = {"blocks": [{"nchan": c, "nbeam": b, "nsamp": t} for c, b, t in shapes]}
+    flop = sum(8.0 * b * 64 * c * 2 * t for c, b, t in shapes)
+    samples = sum(64 * 2 * c * t for c, b, t in shapes)
+    for prec_name, prec in (("fp16", dsaxbf.PREC_FP16), ("fp8split", dsaxbf.PREC_FP8SPLIT)):
+        plans, bufs = [], []
+        for i, (c, b, t) in enumerate(shapes):
+            cfg = dsaxbf.config(nant=64, nchan=c, npkt=t // 2, nbeam=b, tavg=32, precision=prec,
+                                weight_mode=dsaxbf.WEIGHTS_FINE, angle_mode=dsaxbf.ANGLE_SIN)
+            antpos, cal, theta = dsaxbf.synth_geometry(64, c, b, seed + i, 1200.0, 0.1, 1.75)
+            plan = dsaxbf.Plan(cfg)
+            plan.set_weights(antpos, cal, dsaxbf.reference_freqs(1498.75, c), theta)
+            d_in = dsaxbf.DeviceBuffer(dsaxbf.input_bytes(cfg))
+            d_in.upload(dsaxbf.synth_block(cfg, seed + 100 + i))
+            d_sc, d_out, d_bp = (dsaxbf.DeviceBuffer(4 * b), dsaxbf.DeviceBuffer(dsaxbf.output_bytes(cfg)),
+                                 dsaxbf.DeviceBuffer(4 * b))
+            d_sc.upload(np.full(b, 1e-4, np.float32))
+            plans.append(plan)
+            bufs.append((d_in, d_sc, d_out, d_bp))
+        stream = plans[0].stream
+        # one execute per block, default tile shape, same stream
+        for p_, (d_in, d_sc, d_out, d_bp) in zip(plans, bufs):
+            p_.execute(d_in, d_sc, d_out, None, d_bp, stream)
+        ev.sync()
+        e0, e1 = ev.create(2)
+        ev.record(e0, stream)
+        for _ in range(reps):
+            for p_, (d_in, d_sc, d_out, d_bp) in zip(plans, bufs):
+                p_.execute(d_in, d_sc, d_out, None, d_bp, stream)
+        ev.record(e1, stream)
+        seq_ms = ev.ms(e0, e1) / reps
+        batch = dsaxbf.Batch(plans, *[[bb[k] for bb in bufs] for k in range(4)])
+
+        def replay_ms():
+            batch.launch(stream)
+            ev.sync()
+            a0, a1 = ev.create(2)
+            ev.record(a0, stream)
+            for _ in range(reps):
+                batch.launch(stream)
+            ev.record(a1, stream)
+            return ev.ms(a0, a1) / reps
+        default_ms = replay_ms()
+        t0 = time.time()
+        batch.autotune(3)
+        tune_s = time.time() - t0
+        tuned_ms = replay_ms()
+        picks = [dsaxbf.VARIANT_NAMES[p_.variant] for p_ in plans]
+        res[prec_name] = {
+            "graph_ms": tuned_ms, "graph_ms_default_tiles": default_ms, "sequential_ms": seq_ms,
+            "samples_per_s": samples / (tuned_ms / 1e3), "tflops": flop / (tuned_ms / 1e3) / 1e12,
+            "mfma_frac": flop / (tuned_ms / 1e3) / 1e12 / PEAK_TFLOPS[prec_name],
+            "speedup_vs_sequential": seq_ms / tuned_ms, "kernels_per_launch": batch.kernels,
+            "autotune_s": tune_s,
+            "tile_picks": {v: picks.count(v) for v in sorted(set(picks))}}
+        batch.close()
+        for p_ in plans:
+            p_.close()
+        for bb in bufs:
+            for x in bb:
+                x.free()
+    res["flop_per_batch"] = flop
+    return res
 
 
 def main():
@@ -270,8 +345,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the oracle error report")
     ap.add_argument("--check-groups", type=int, default=8)
-    ap.add_argument("--schedule", choices=["auto", "classic", "pipelined"], default="auto",
-                    help="fused-kernel schedule (auto: the library's autotuner picks)")
+    ap.add_argument("--variant", default="auto",
+                    choices=["auto", "classic256", "pipelined256", "classic128s", "classic128",
+                             "classic64"],
+                    help="fused-kernel variant (auto: the library's autotuner picks)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched uneven-blocks measurement (one graph launch)")
     ap.add_argument("--no-stream", action="store_true",
                     help="skip the streamed scaled-array measurement (256 antennas, 1024 beams)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -301,9 +380,9 @@ def main():
     res = {}
     for prec in ("fp16", "fp8split"):
         res[prec] = time_path(dsaxbf, ev, prec, cfg, antpos, cal, theta, freqs, blocks,
-                              args.steps, args.warmup, args.schedule)
+                              args.steps, args.warmup, args.variant)
         print(f"# {prec}: {res[prec]['ms_per_step']:.4f} ms/step, kernel "
-              f"{res[prec]['kernel_ms']:.4f} ms, schedule {res[prec]['schedule']} "
+              f"{res[prec]['kernel_ms']:.4f} ms, variant {res[prec]['variant']} "
               f"(autotune ms {res[prec]['autotune_ms']})", file=sys.stderr, flush=True)
 
     errs = {}
@@ -334,6 +413,15 @@ def main():
             stream = {"error": f"{type(exc).__name__}: {exc}"}
         print(f"# stream: {stream}", file=sys.stderr, flush=True)
 
+    batch = None
+    if not args.no_batch:
+        try:
+            batch = batch_bench(dsaxbf, ev)
+        except Exception as exc:             # reported, never fatal to the headline line
+            batch = {"error": f"{type(exc).__name__}: {exc}"}
+        print(f"# batch: { {k: v for k, v in batch.items() if k != 'blocks'} }", file=sys.stderr,
+              flush=True)
+
     main_p = args.precision
     r = res[main_p]
     value = SAMPLES_PER_BLOCK / (r["ms_per_step"] / 1e3)
@@ -354,7 +442,8 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[main_p],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[main_p],
                      "traffic": traffic_from_profiles(main_p),
-                     "kernel": f"bf_{r['schedule']}_kernel", "kernel_ms": r["kernel_ms"],
+                     "kernel": "bf_pipelined_kernel" if r["variant"] == "pipelined256"
+                     else "bf_classic_kernel", "variant": r["variant"], "kernel_ms": r["kernel_ms"],
                      "flop_per_launch": FLOP_PER_BLOCK},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
                                                   ("value", "unit", "cores", "kind", "sample")},
@@ -364,11 +453,12 @@ def main():
                       / PEAK_TFLOPS[p],
                       "executed_mfma_frac": (2.0 if p == "fp8split" else 1.0) * FLOP_PER_BLOCK
                       / (res[p]["kernel_ms"] / 1e3) / 1e12 / PEAK_TFLOPS[p],
-                      "schedule": res[p]["schedule"], "autotune_ms": res[p]["autotune_ms"],
+                      "variant": res[p]["variant"], "autotune_ms": res[p]["autotune_ms"],
                       "errors_vs_fp64_oracle": errs.get(p)} for p in res},
         "fp8split_speedup_vs_fp16": res["fp16"]["ms_per_step"] / res["fp8split"]["ms_per_step"],
         "tone_check": {k: errs[k] for k in ("tone_beam", "peak_beam_fp16") if k in errs},
         "scaled_array_stream": stream,
+        "uneven_batch_graph": batch,
     }
     print(json.dumps(line), flush=True)
 

@@ -8,20 +8,20 @@
 // pols; integer-valued sums are exact in float.
 //  stuff 1: beam b <- antenna ((b/16)%4)*16 + b%16  (dsaX_beamformer.cu:326-337)
 //  stuff 2: beam 0 <- sum over all antennas          (dsaX_beamformer.cu:307-323)
-// The per-block beam sums in bpart are rewritten for the beams written here.
+// The per-32-sample beam sums in bpart are rewritten for the beams written here.
 __global__ __launch_bounds__(256) void bf_stuff_kernel(bfk_dims d, const uint8_t *__restrict__ in,
                                                        const float *__restrict__ scale,
                                                        uint8_t *__restrict__ out,
                                                        float *__restrict__ power,
                                                        float *__restrict__ bpart) {
-  __shared__ float S[256][8];
-  __shared__ float IB[8];
+  __shared__ float S[256][10];      // 8 bins, then the two 32-sample halves
+  __shared__ float IB[10];
   const int nblk = d.nsamp / 64;
   const int t64 = blockIdx.x % nblk, g = blockIdx.x / nblk;
   const int nbin = 64 / d.tavg < 1 ? 1 : 64 / d.tavg;
   const int ppb = d.tavg / 2 < 32 ? d.tavg / 2 : 32;      // packets per bin
   const int tid = threadIdx.x;
-  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float s[10] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if (tid < d.nant) {
     for (int pl = 0; pl < 32; pl++) {
       const long p = (long)t64 * 32 + pl;
@@ -42,12 +42,14 @@ __global__ __launch_bounds__(256) void bf_stuff_kernel(bfk_dims d, const uint8_t
 #pragma unroll
       for (int k = 0; k < 8; k++)
         if (k == bin) s[k] += acc;
+      if (pl < 16) s[8] += acc;
+      else s[9] += acc;
     }
   }
 #pragma unroll
-  for (int k = 0; k < 8; k++) S[tid][k] = s[k];
+  for (int k = 0; k < 10; k++) S[tid][k] = s[k];
   __syncthreads();
-  if (tid < nbin) {
+  if (tid < 10) {
     float x = 0.f;
     for (int a = 0; a < d.nant; a++) x += S[a][tid];
     IB[tid] = x;
@@ -56,15 +58,15 @@ __global__ __launch_bounds__(256) void bf_stuff_kernel(bfk_dims d, const uint8_t
   const int bend = d.stuff == 2 ? 1 : d.nbeam;
   for (int b = tid; b < bend; b += 256) {
     const int ant = ((b / 16) % 4) * 16 + b % 16;
-    float tile = 0.f;
     for (int k = 0; k < nbin; k++) {
       const float x = d.stuff == 2 ? IB[k] : (ant < d.nant ? S[ant][k] : 0.f);
       const long idx = ((long)(t64 * nbin + k) * d.nbeam + b) * d.ngroup + g;
       if (power) power[idx] = x;
       out[idx] = requant_u8(x, scale[b]);
-      tile += x;
     }
-    bpart[((long)b * nblk + t64) * d.ngroup + g] = tile;
+    for (int hh = 0; hh < 2; hh++)
+      bpart[((long)b * 2 * nblk + 2 * t64 + hh) * d.ngroup + g] =
+          d.stuff == 2 ? IB[8 + hh] : (ant < d.nant ? S[ant][8 + hh] : 0.f);
   }
 }
 
@@ -75,7 +77,7 @@ __global__ __launch_bounds__(256) void bf_beam_power_kernel(bfk_dims d, const fl
                                                             float *__restrict__ bp) {
   __shared__ double red[256];
   const int b = blockIdx.x, tid = threadIdx.x;
-  const long n = (long)(d.nsamp / 64) * d.ngroup;
+  const long n = (long)(d.nsamp / 32) * d.ngroup;
   const float *src = bpart + (long)b * n;
   double s = 0.0;
   for (long i = tid; i < n; i += 256) s += src[i];
@@ -100,7 +102,7 @@ extern "C" int bfk_stuff(const bfk_dims *d, const uint8_t *d_in, const float *d_
 }
 
 extern "C" size_t bfk_bpart_bytes(const bfk_dims *d) {
-  return (size_t)d->nbeam * (d->nsamp / 64) * d->ngroup * sizeof(float);
+  return (size_t)d->nbeam * (d->nsamp / 32) * d->ngroup * sizeof(float);
 }
 
 extern "C" int bfk_beam_power(const bfk_dims *d, const float *d_bpart, float *d_beam_power,

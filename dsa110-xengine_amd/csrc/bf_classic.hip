@@ -1,7 +1,8 @@
 // bf_classic.hip -- the "classic" schedule of the fused beamformer: one tile
 // per workgroup, two workgroups resident per CU (at most 80 KB of LDS and 256
 // registers per wave each), so that one workgroup's staging and epilogue
-// overlap the other's MFMA work.
+// overlap the other's MFMA work.  Also the batched form: the tiles of many
+// blocks (each with its own shape, weights and buffers) in one launch.
 //
 // Reference path: src/dsaX_beamformer.cu promoter (:170-193), beamformer
 // (:267-416), badder (:78-112) and adder (:120-158), fused into one launch.
@@ -14,18 +15,24 @@
 //   (c) |B|^2, the sums over pols, channels and tavg samples, scaling and the
 //       uint8 requantisation in registers.
 //
+// Tile shapes.  A wave owns NB 32-beam tiles.  Without a split the 4 waves take
+// 4 x NB x 32 beams and both 32-sample units of the tile; with a split, wave
+// pairs share NB x 32 beams and take one unit each, so a tile has half the
+// beams (256, 128 or 64 beams per tile over NB = 2, 1 and the split).  Every
+// shape runs the same MFMA chains and sums in the same order, so all give the
+// same bits; which is fastest depends on the block (few beams favour small
+// tiles).
+//
 // Arrays of more than 64 antennas (NKC = 2 or 4 chunks of 64) keep the LDS
 // stage at the same size by staging the group's channels in NKC sub-steps of
 // 8/NKC channels, each row holding all antennas; a wave then owns one 32-beam
-// tile so that its weights (all NKC chunks) still fit in registers.
+// tile (NB = 1) so that its weights (all NKC chunks) still fit in registers.
 #include "bf_common.h"
 
 namespace {
 constexpr int NW = 4, NT = NW * 64;       // waves, threads per workgroup
 
 template <int NKC> struct Shape {
-  static constexpr int NB = NKC == 1 ? BTW : 1;               // beam tiles per wave
-  static constexpr int BSET = NW * NB * 32;                   // beams per tile
   static constexpr int ROWB = NKC == 1 ? 80 : 64 * NKC + 16;  // staged row stride
   static constexpr int NCI = 8 / NKC;                         // channels per sub-step
   static constexpr int CQS = 2 * NCI;                         // (channel, pol) per sub-step
@@ -34,29 +41,25 @@ template <int NKC> struct Shape {
 };
 }
 
-template <int PREC, int TAVG, int NKC>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
-void bf_classic_kernel(BfArgs a) {
+// One tile L of job a (all threads of the workgroup).
+template <int PREC, int TAVG, int NKC, int NB>
+__device__ __forceinline__ void classic_tile(const BfArgs &a, int L, uint8_t *stage) {
   using S = Shape<NKC>;
-  constexpr int NB = S::NB, ROWB_ = S::ROWB, NCI = S::NCI, CQS = S::CQS;
-  __shared__ __attribute__((aligned(16))) uint8_t stage[S::STAGE];
+  constexpr int ROWB_ = S::ROWB, NCI = S::NCI, CQS = S::CQS;
   constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;
-  constexpr int NTT = TAVG == 64 ? 1 : 2;
-
-  // XCD-aware order: workgroup P runs on XCD P % 8; consecutive tiles L go
-  // to one XCD (tile_id explains the order).
-  const int P = blockIdx.x;
-  const int L = (P & 7) * (gridDim.x >> 3) + (P >> 3);
-  if (L >= a.ntiles) return;
   const TileId t = tile_id(a, L);
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int n = lane & 31, h = lane >> 5;
-  const int btw = t.bset * (S::BSET / 32) + wave * NB;   // first beam tile of this wave
+  const int split = a.split;
+  const int nbq = NW >> split;                           // beam groups per tile
+  const int bq = wave % nbq, th = wave / nbq;            // th: unit of a split pair
+  const int btw = (t.bset * nbq + bq) * NB;              // first beam tile of this wave
+  const int tt_first = split ? th : 0, ntt = split ? 1 : 2;
 
-  float part[NTT][NB][NP];
+  float part[2][NB][NP];             // per unit this wave runs
 #pragma unroll
-  for (int i = 0; i < NTT; i++)
+  for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int bt = 0; bt < NB; bt++)
 #pragma unroll
@@ -95,64 +98,156 @@ void bf_classic_kernel(BfArgs a) {
     }
     __syncthreads();
 
-    // (b) CQS (channel, pol) GEMMs, two 32-sample units each
+    // (b) CQS (channel, pol) GEMMs, one or two 32-sample units each
     for (int cql = 0; cql < CQS; cql++) {
       const long wr = weight_row(a, t.g, sub * CQS + cql);
       WFrag<PREC, NB, NKC> W;
       W.load(a.wfrag, a.wscale, wr * a.nbt + btw, wr * a.nbeam_pad + btw * 32, lane, n);
 #pragma unroll
-      for (int tt = 0; tt < 2; tt++) {
-        Acc<NB> A;
-        gemm_unit(A, W, stage + (cql * TCU + tt * 32) * ROWB_, ROWB_, n, h);
-        detect<TAVG>(A, W.sc, part[NTT == 1 ? 0 : tt]);
+      for (int k = 0; k < 2; k++) {
+        if (k < ntt) {
+          Acc<NB> A;
+          gemm_unit(A, W, stage + (cql * TCU + (tt_first + k) * 32) * ROWB_, ROWB_, n, h);
+          detect<TAVG>(A, W.sc, part[k]);
+        }
       }
     }
   }
 
-  // (c)
-  tile_output<TAVG>(a, t, btw, n, h, part);
+  // (c) a split TAVG-64 bin needs both units: the second wave of each pair
+  // hands its half to the first through LDS
+  if (TAVG == 64 && split) {
+    float *xch = (float *)stage;
+    __syncthreads();                   // every wave is done with the stage
+    if (th == 1)
+#pragma unroll
+      for (int bt = 0; bt < NB; bt++) xch[(bq * NB + bt) * 64 + lane] = part[0][bt][0];
+    __syncthreads();
+    if (th == 0) {
+#pragma unroll
+      for (int bt = 0; bt < NB; bt++) part[1][bt][0] = xch[(bq * NB + bt) * 64 + lane];
+      tile_output<TAVG>(a, t, btw, n, h, part, 0, 2);
+    }
+    return;
+  }
+  tile_output<TAVG>(a, t, btw, n, h, part, tt_first, ntt);
 }
 
-template <int PREC, int NKC>
-static void launch(int tavg, dim3 grid, hipStream_t s, const BfArgs &a) {
-  switch (tavg) {
-    case 8:  hipLaunchKernelGGL((bf_classic_kernel<PREC, 8, NKC>), grid, dim3(NT), 0, s, a); break;
-    case 16: hipLaunchKernelGGL((bf_classic_kernel<PREC, 16, NKC>), grid, dim3(NT), 0, s, a); break;
-    case 32: hipLaunchKernelGGL((bf_classic_kernel<PREC, 32, NKC>), grid, dim3(NT), 0, s, a); break;
-    default: hipLaunchKernelGGL((bf_classic_kernel<PREC, 64, NKC>), grid, dim3(NT), 0, s, a); break;
+template <int PREC, int TAVG, int NKC, int NB>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void bf_classic_kernel(BfArgs a) {
+  __shared__ __attribute__((aligned(16))) uint8_t stage[Shape<NKC>::STAGE];
+  // XCD-aware order: workgroup P runs on XCD P % 8; consecutive tiles L go
+  // to one XCD (tile_id explains the order).
+  const int P = blockIdx.x;
+  const int L = (P & 7) * (gridDim.x >> 3) + (P >> 3);
+  if (L >= a.ntiles) return;
+  classic_tile<PREC, TAVG, NKC, NB>(a, L, stage);
+}
+
+// Batched: the tiles of njobs blocks back to back; job j owns global tiles
+// [tile_start[j], tile_start[j+1]).
+template <int PREC, int TAVG, int NKC, int NB>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void bf_classic_batch_kernel(const BfArgs *__restrict__ jobs, const int *__restrict__ tile_start,
+                             int njobs) {
+  __shared__ __attribute__((aligned(16))) uint8_t stage[Shape<NKC>::STAGE];
+  const int P = blockIdx.x;
+  const int L = (P & 7) * (gridDim.x >> 3) + (P >> 3);
+  if (L >= tile_start[njobs]) return;
+  int j = 0;
+  while (L >= tile_start[j + 1]) j++;
+  classic_tile<PREC, TAVG, NKC, NB>(jobs[j], L - tile_start[j], stage);
+}
+
+// ---- host side --------------------------------------------------------------
+
+// NB and split of a classic variant; 0 if the variant is not a classic one.
+static int classic_shape(int variant, int nkc, int *nb, int *split) {
+  switch (variant) {
+    case BFK_VAR_CLASSIC_256: *nb = 2; *split = 0; return nkc == 1;
+    case BFK_VAR_CLASSIC_128S: *nb = 2; *split = 1; return nkc == 1;
+    case BFK_VAR_CLASSIC_128: *nb = 1; *split = 0; return 1;
+    case BFK_VAR_CLASSIC_64: *nb = 1; *split = 1; return 1;
+    default: return 0;
   }
 }
 
-template <int NKC>
-static int beamform(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag, const float *d_wscale,
-                    const float *d_scale, uint8_t *d_out, float *d_power, float *d_bpart,
-                    hipStream_t s) {
-  if (d->nbeam_pad % Shape<NKC>::BSET != 0 || d->nsamp % TCU != 0) {
-    bfk_set_error("bfk_beamform: unsupported shape nbeam_pad=%d nsamp=%d", d->nbeam_pad, d->nsamp);
+extern "C" int bfk_classic_beams(int variant, int nkc) {
+  int nb, split;
+  if (!classic_shape(variant, nkc, &nb, &split)) return 0;
+  return (NW >> split) * nb * 32;
+}
+
+template <int PREC, int NKC, int NB>
+static void launch(int tavg, dim3 grid, hipStream_t s, const BfArgs *a, const BfArgs *jobs,
+                   const int *tile_start, int njobs) {
+#define BF_LAUNCH(T)                                                                                   \
+  do {                                                                                                 \
+    if (a) hipLaunchKernelGGL((bf_classic_kernel<PREC, T, NKC, NB>), grid, dim3(NT), 0, s, *a);      \
+    else hipLaunchKernelGGL((bf_classic_batch_kernel<PREC, T, NKC, NB>), grid, dim3(NT), 0, s, jobs, \
+                            tile_start, njobs);                                                        \
+  } while (0)
+  switch (tavg) {
+    case 8: BF_LAUNCH(8); break;
+    case 16: BF_LAUNCH(16); break;
+    case 32: BF_LAUNCH(32); break;
+    default: BF_LAUNCH(64); break;
+  }
+#undef BF_LAUNCH
+}
+
+// Launch either one job (a != NULL) or a batch (device tables) of one
+// instantiation (prec, tavg, nkc, nb).
+static int dispatch(int prec, int tavg, int nkc, int nb, dim3 grid, hipStream_t s, const BfArgs *a,
+                    const BfArgs *jobs, const int *tile_start, int njobs) {
+  const int key = (nkc << 2) | (nb << 1) | prec;
+  switch (key) {
+    case (1 << 2) | (2 << 1) | 0: launch<0, 1, 2>(tavg, grid, s, a, jobs, tile_start, njobs); break;
+    case (1 << 2) | (2 << 1) | 1: launch<1, 1, 2>(tavg, grid, s, a, jobs, tile_start, njobs); break;
+    case (1 << 2) | (1 << 1) | 0: launch<0, 1, 1>(tavg, grid, s, a, jobs, tile_start, njobs); break;
+    case (1 << 2) | (1 << 1) | 1: launch<1, 1, 1>(tavg, grid, s, a, jobs, tile_start, njobs); break;
+    case (2 << 2) | (1 << 1) | 0: launch<0, 2, 1>(tavg, grid, s, a, jobs, tile_start, njobs); break;
+    case (2 << 2) | (1 << 1) | 1: launch<1, 2, 1>(tavg, grid, s, a, jobs, tile_start, njobs); break;
+    case (4 << 2) | (1 << 1) | 0: launch<0, 4, 1>(tavg, grid, s, a, jobs, tile_start, njobs); break;
+    case (4 << 2) | (1 << 1) | 1: launch<1, 4, 1>(tavg, grid, s, a, jobs, tile_start, njobs); break;
+    default:
+      bfk_set_error("bfk_beamform: no classic kernel for %d antenna chunks x %d beam tiles", nkc, nb);
+      return -1;
+  }
+  return bfk_check(hipGetLastError(), "bf_classic_kernel");
+}
+
+extern "C" int bfk_classic_args(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
+                                const float *d_wscale, const float *d_scale, uint8_t *d_out,
+                                float *d_power, float *d_bpart, BfArgs *a, int *nb) {
+  int split;
+  if (d->nant_pad != 64 * d->nkc || (d->nkc != 1 && d->nkc != 2 && d->nkc != 4) ||
+      !classic_shape(d->variant, d->nkc, nb, &split) || d->nsamp % TCU != 0) {
+    bfk_set_error("bfk_beamform: variant %d cannot run nant_pad=%d nsamp=%d", d->variant, d->nant_pad,
+                  d->nsamp);
     return -1;
   }
-  const BfArgs a = bf_make_args(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart,
-                                Shape<NKC>::BSET);
-  const dim3 grid((a.ntiles + 7) / 8 * 8);
-  if (d->prec == 0) launch<0, NKC>(d->tavg, grid, s, a);
-  else launch<1, NKC>(d->tavg, grid, s, a);
-  return bfk_check(hipGetLastError(), "bf_classic_kernel");
+  *a = bf_make_args(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart,
+                    (NW >> split) * *nb * 32);
+  a->split = split;
+  return 0;
 }
 
 extern "C" int bfk_beamform_classic(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
                                     const float *d_wscale, const float *d_scale, uint8_t *d_out,
                                     float *d_power, float *d_bpart, void *stream) {
-  hipStream_t s = (hipStream_t)stream;
-  if (d->nant_pad != 64 * d->nkc) {
-    bfk_set_error("bfk_beamform: nant_pad=%d is not 64*nkc", d->nant_pad);
-    return -1;
-  }
-  switch (d->nkc) {
-    case 1: return beamform<1>(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, s);
-    case 2: return beamform<2>(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, s);
-    case 4: return beamform<4>(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, s);
-    default:
-      bfk_set_error("bfk_beamform: %d antenna chunks unsupported (1, 2 or 4)", d->nkc);
-      return -1;
-  }
+  BfArgs a;
+  int nb;
+  if (bfk_classic_args(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, &a, &nb)) return -1;
+  const dim3 grid((a.ntiles + 7) / 8 * 8);
+  return dispatch(d->prec, d->tavg, d->nkc, nb, grid, (hipStream_t)stream, &a, NULL, NULL, 0);
+}
+
+extern "C" int bfk_beamform_classic_batch(int prec, int tavg, int nkc, int nb, const void *d_jobs,
+                                          const int *d_tile_start, int njobs, int ntiles,
+                                          void *stream) {
+  const dim3 grid((ntiles + 7) / 8 * 8);
+  return dispatch(prec, tavg, nkc, nb, grid, (hipStream_t)stream, NULL, (const BfArgs *)d_jobs,
+                  d_tile_start, njobs);
 }

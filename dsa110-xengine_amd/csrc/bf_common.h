@@ -132,7 +132,8 @@ struct BfArgs {
   const float *scale;
   uint8_t *out;
   float *power;   // optional
-  float *bpart;   // [nbeam][nsamp/64][ngroup] per-tile beam power sums
+  float *bpart;   // [nbeam][nsamp/32][ngroup] per-unit beam power sums
+  int split;      // classic: wave pairs split a tile's two 32-sample units
   int nant, nchan, ngroup, nbeam, nbeam_pad, nbt, wmode, order;
   int ntblk, nbsets, ntiles, tiles_per_xcd;
   uint32_t pstride, astride, gstride;     // input bytes per packet, antenna, channel group
@@ -369,33 +370,47 @@ __device__ __forceinline__ void stage_store(uint8_t *stage, const uint32_t (&v)[
   }
 }
 
-// The tile's outputs: lanes h = 0 and h = 1 hold the two row halves of each
-// bin; the tile's sum per beam (the AGC input) goes to bpart.
+// The tile's outputs.  part[k] holds this wave's bins of 32-sample unit
+// tt_first + k (k < ntt); lanes h = 0 and h = 1 hold the two row halves of each
+// bin.  TAVG 64 has one bin per tile, the sum of both units' halves, so it
+// needs ntt = 2.  Each unit's sum per beam (the AGC input) goes to bpart
+// [nbeam][nsamp/32][ngroup].
 template <int TAVG, int NB>
 __device__ __forceinline__ void tile_output(const BfArgs &a, const TileId &t, int btw, int n, int h,
-                                            float (&part)[TAVG == 64 ? 1 : 2][NB]
-                                                         [TAVG <= 32 ? 32 / TAVG : 1]) {
+                                            float (&part)[2][NB][TAVG <= 32 ? 32 / TAVG : 1],
+                                            int tt_first, int ntt) {
   constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;
-  constexpr int NTT = TAVG == 64 ? 1 : 2;
 #pragma unroll
   for (int bt = 0; bt < NB; bt++) {
     const int b = (btw + bt) * 32 + n;
-    float tile_sum = 0.f;
+    const bool own = h == 0 && b < a.nbeam;
+    float x[2][NP];
 #pragma unroll
-    for (int tt = 0; tt < NTT; tt++)
+    for (int k = 0; k < 2; k++) {
+      if (k >= ntt) continue;
+      float unit_sum = 0.f;
 #pragma unroll
       for (int j = 0; j < NP; j++) {
-        float x = part[tt][bt][j];
-        x += __shfl_xor(x, 32, 64);
-        tile_sum += x;
-        if (h == 0 && b < a.nbeam) {
-          const int o = TAVG == 64 ? t.tblk : t.tblk * (TCU / TAVG) + tt * (32 / TAVG) + j;
-          const long idx = ((long)o * a.nbeam + b) * a.ngroup + t.g;
-          if (a.power) a.power[idx] = x;
-          a.out[idx] = requant_u8(x, a.scale[b]);
-        }
+        x[k][j] = part[k][bt][j] + __shfl_xor(part[k][bt][j], 32, 64);
+        unit_sum += x[k][j];
       }
-    if (h == 0 && b < a.nbeam) a.bpart[((long)b * a.ntblk + t.tblk) * a.ngroup + t.g] = tile_sum;
+      const int tt = tt_first + k;
+      if (own) a.bpart[((long)b * 2 * a.ntblk + 2 * t.tblk + tt) * a.ngroup + t.g] = unit_sum;
+      if (TAVG < 64 && own)
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+          const int o = t.tblk * (TCU / TAVG) + tt * (32 / TAVG) + j;
+          const long idx = ((long)o * a.nbeam + b) * a.ngroup + t.g;
+          if (a.power) a.power[idx] = x[k][j];
+          a.out[idx] = requant_u8(x[k][j], a.scale[b]);
+        }
+    }
+    if (TAVG == 64 && own) {
+      const float y = x[0][0] + x[1][0];
+      const long idx = ((long)t.tblk * a.nbeam + b) * a.ngroup + t.g;
+      if (a.power) a.power[idx] = y;
+      a.out[idx] = requant_u8(y, a.scale[b]);
+    }
   }
 }
 

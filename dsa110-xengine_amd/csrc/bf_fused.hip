@@ -92,7 +92,6 @@ void bf_pipelined_kernel(BfArgs a) {
   __shared__ __attribute__((aligned(16))) uint8_t stage[STAGE_BYTES];
   __shared__ __attribute__((aligned(16))) uint8_t raw[RAW_BYTES];
   constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;   // bins per 32-row unit
-  constexpr int NTT = TAVG == 64 ? 1 : 2;          // row units with their own bins
   static_assert(RAW_LOADS == 16, "raw slices are issued two per (channel, pol) pair");
 
   // Persistent, XCD-aware: workgroup P runs on XCD P % 8 and walks that XCD's
@@ -146,9 +145,9 @@ void bf_pipelined_kernel(BfArgs a) {
       return (uint32_t)weight_row(a, t.g, cq) * a.nbeam_pad + btw * 32;
     };
 
-    float part[NTT][BTW][NP];
+    float part[2][BTW][NP];          // per 32-sample unit
 #pragma unroll
-    for (int i = 0; i < NTT; i++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
       for (int bt = 0; bt < BTW; bt++)
 #pragma unroll
@@ -182,7 +181,7 @@ void bf_pipelined_kernel(BfArgs a) {
       W0.load_buf(rw, rs, wrow_bt(cq + 1), sc_off(cq + 1), vlane16, vn4);
       slice = raw_fetch(rin, to_next, a, wave, voff_even, voff_odd, cq);
       gemm_unit(A0, W1, stage + (cq * TCU) * ROWB, ROWB, n, h);                // unit 2cq
-      detect<TAVG>(A1, sc_prev, part[NTT - 1]);                                 // unit 2cq-1
+      detect<TAVG>(A1, sc_prev, part[1]);                                 // unit 2cq-1
       __builtin_amdgcn_sched_barrier(0);
       gemm_unit(A1, W1, stage + (cq * TCU + 32) * ROWB, ROWB, n, h);           // unit 2cq+1
       detect<TAVG>(A0, W1.sc, part[0]);                                          // unit 2cq
@@ -194,7 +193,7 @@ void bf_pipelined_kernel(BfArgs a) {
       W1.load_buf(rw, rs, wrow_bt(cq + 2), sc_off(cq + 2), vlane16, vn4);
       slice = raw_fetch(rin, to_next, a, wave, voff_even, voff_odd, cq + 1);
       gemm_unit(A0, W0, stage + ((cq + 1) * TCU) * ROWB, ROWB, n, h);          // unit 2cq+2
-      detect<TAVG>(A1, sc_prev, part[NTT - 1]);                                 // unit 2cq+1
+      detect<TAVG>(A1, sc_prev, part[1]);                                 // unit 2cq+1
       __builtin_amdgcn_sched_barrier(0);
       gemm_unit(A1, W0, stage + ((cq + 1) * TCU + 32) * ROWB, ROWB, n, h);     // unit 2cq+3
       detect<TAVG>(A0, W0.sc, part[0]);                                          // unit 2cq+2
@@ -206,15 +205,15 @@ void bf_pipelined_kernel(BfArgs a) {
     raw_put(raw, wave, lane, 14, slice);
     slice = raw_fetch(rin, to_next, a, wave, voff_even, voff_odd, 15);
     gemm_unit(A0, W1, stage + (15 * TCU) * ROWB, ROWB, n, h);                   // unit 30
-    detect<TAVG>(A1, sc_prev, part[NTT - 1]);                                   // unit 29
+    detect<TAVG>(A1, sc_prev, part[1]);                                   // unit 29
     __builtin_amdgcn_sched_barrier(0);
     gemm_unit(A1, W1, stage + (15 * TCU + 32) * ROWB, ROWB, n, h);              // unit 31
     detect<TAVG>(A0, W1.sc, part[0]);                                           // unit 30
     __builtin_amdgcn_sched_barrier(0);
-    detect<TAVG>(A1, W1.sc, part[NTT - 1]);                                     // unit 31
+    detect<TAVG>(A1, W1.sc, part[1]);                                     // unit 31
     raw_put(raw, wave, lane, 15, slice);
 
-    tile_output<TAVG>(a, t, btw, n, h, part);
+    tile_output<TAVG>(a, t, btw, n, h, part, 0, 2);
     L = Lnext;
   }
 }
@@ -249,7 +248,7 @@ BfArgs bf_make_args(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
   a.wmode = d->wmode;
   a.order = d->order;
   a.ntblk = d->nsamp / TCU;
-  a.nbsets = d->nbeam_pad / bset_beams;
+  a.nbsets = (d->nbeam + bset_beams - 1) / bset_beams;   // beams past nbeam: zero weights, no output
   a.ntiles = a.nbsets * d->ngroup * a.ntblk;
   a.tiles_per_xcd = (a.ntiles + 7) / 8;
   a.pstride = (uint32_t)d->nant * d->nchan * 4;
@@ -258,6 +257,7 @@ BfArgs bf_make_args(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
   a.in_bytes = (uint32_t)((size_t)d->npkt * d->nant * d->nchan * 4);
   a.wfrag_bytes = (uint32_t)bfk_wfrag_bytes(d);
   a.wscale_bytes = (uint32_t)bfk_wscale_bytes(d);
+  a.split = 0;
   return a;
 }
 
@@ -265,10 +265,12 @@ extern "C" int bfk_beamform_classic(const bfk_dims *d, const uint8_t *d_in, cons
                                     const float *d_wscale, const float *d_scale, uint8_t *d_out,
                                     float *d_power, float *d_bpart, void *stream);
 
-extern "C" int bfk_sched_supported(const bfk_dims *d, int sched) {
-  if (sched == BFK_SCHED_CLASSIC) return d->nkc == 1 || d->nkc == 2 || d->nkc == 4;
-  if (sched == BFK_SCHED_PIPELINED) return d->nkc == 1 && d->nbeam_pad % BSET == 0;
-  return 0;
+extern "C" int bfk_classic_beams(int variant, int nkc);
+
+extern "C" int bfk_variant_supported(const bfk_dims *d, int v) {
+  if (d->nant_pad != 64 * d->nkc || (d->nkc != 1 && d->nkc != 2 && d->nkc != 4)) return 0;
+  if (v == BFK_VAR_PIPELINED_256) return d->nkc == 1;
+  return bfk_classic_beams(v, d->nkc) > 0;
 }
 
 static int beamform_pipelined(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
@@ -304,13 +306,11 @@ extern "C" int bfk_beamform(const bfk_dims *d, const uint8_t *d_in, const void *
     bfk_set_error("bfk_beamform: block or weights above 2 GiB");
     return -1;
   }
-  switch (d->sched) {
-    case BFK_SCHED_CLASSIC:
-      return bfk_beamform_classic(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, stream);
-    case BFK_SCHED_PIPELINED:
-      return beamform_pipelined(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, stream);
-    default:
-      bfk_set_error("bfk_beamform: unknown schedule %d", d->sched);
-      return -1;
+  if (!bfk_variant_supported(d, d->variant)) {
+    bfk_set_error("bfk_beamform: variant %d cannot run this shape (nant_pad %d)", d->variant, d->nant_pad);
+    return -1;
   }
+  if (d->variant == BFK_VAR_PIPELINED_256)
+    return beamform_pipelined(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, stream);
+  return bfk_beamform_classic(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, stream);
 }

@@ -1,0 +1,116 @@
+// bf_jobs.hip -- many blocks in one launch per kernel instantiation.
+//
+// The reference beamforms one fixed-shape block at a time (five launches per
+// 32-sample int, dsaX_beamformer.cu:955-1003).  A batch here holds blocks of
+// different channel, beam and time counts, each with its own weights, buffers
+// and tile shape (variant).  Blocks that share a classic kernel instantiation
+// (precision, tavg, antenna chunks, beam tiles per wave) go into one launch
+// whose workgroups walk a device table of jobs; every job keeps its own
+// BfArgs, so shapes mix freely.  bfk_batch_launch allocates nothing and never
+// synchronises with the host, so a caller can capture it in a HIP graph.
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "bf_common.h"
+
+extern "C" int bfk_classic_args(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
+                                const float *d_wscale, const float *d_scale, uint8_t *d_out,
+                                float *d_power, float *d_bpart, BfArgs *a, int *nb);
+extern "C" int bfk_beamform_classic_batch(int prec, int tavg, int nkc, int nb, const void *d_jobs,
+                                          const int *d_tile_start, int njobs, int ntiles,
+                                          void *stream);
+
+namespace {
+struct Group {
+  int prec, tavg, nkc, nb;
+  int njobs, ntiles;
+  BfArgs *d_jobs;
+  int *d_tile_start;     // njobs + 1 prefix sums
+};
+struct Batch {
+  std::vector<Group> groups;
+};
+}
+
+extern "C" void bfk_batch_free(void *handle) {
+  Batch *b = (Batch *)handle;
+  if (!b) return;
+  for (Group &g : b->groups) {
+    (void)hipFree(g.d_jobs);
+    (void)hipFree(g.d_tile_start);
+  }
+  delete b;
+}
+
+extern "C" int bfk_batch_prepare(const bfk_dims *dims, const bfk_batch_bufs *bufs, int n, void **handle) {
+  *handle = nullptr;
+  if (n < 1) {
+    bfk_set_error("bfk_batch_prepare: empty batch");
+    return -1;
+  }
+  struct Job {
+    int prec, tavg, nkc, nb;
+    BfArgs a;
+  };
+  std::vector<Job> jobs(n);
+  for (int i = 0; i < n; i++) {
+    const bfk_dims *d = &dims[i];
+    if (d->variant == BFK_VAR_PIPELINED_256) {
+      bfk_set_error("bfk_batch_prepare: block %d uses the pipelined variant; batches run classic tiles", i);
+      return -1;
+    }
+    Job &j = jobs[i];
+    if (bfk_classic_args(d, bufs[i].d_in, bufs[i].d_wfrag, bufs[i].d_wscale, bufs[i].d_scale,
+                         bufs[i].d_out, bufs[i].d_power, bufs[i].d_bpart, &j.a, &j.nb))
+      return -1;
+    j.prec = d->prec;
+    j.tavg = d->tavg;
+    j.nkc = d->nkc;
+  }
+  Batch *b = new Batch;
+  std::vector<char> done(n, 0);
+  for (int i = 0; i < n; i++) {
+    if (done[i]) continue;
+    Group g{jobs[i].prec, jobs[i].tavg, jobs[i].nkc, jobs[i].nb, 0, 0, nullptr, nullptr};
+    std::vector<BfArgs> tab;
+    std::vector<int> start{0};
+    for (int k = i; k < n; k++) {
+      const Job &j = jobs[k];
+      if (done[k] || j.prec != g.prec || j.tavg != g.tavg || j.nkc != g.nkc || j.nb != g.nb) continue;
+      done[k] = 1;
+      tab.push_back(j.a);
+      start.push_back(start.back() + j.a.ntiles);
+    }
+    g.njobs = (int)tab.size();
+    g.ntiles = start.back();
+    hipError_t e = hipMalloc((void **)&g.d_jobs, sizeof(BfArgs) * tab.size());
+    if (e == hipSuccess) e = hipMalloc((void **)&g.d_tile_start, sizeof(int) * start.size());
+    if (e == hipSuccess)
+      e = hipMemcpy(g.d_jobs, tab.data(), sizeof(BfArgs) * tab.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(g.d_tile_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice);
+    b->groups.push_back(g);
+    if (e != hipSuccess) {
+      bfk_batch_free(b);
+      return bfk_check(e, "bfk_batch_prepare");
+    }
+  }
+  *handle = b;
+  return 0;
+}
+
+extern "C" int bfk_batch_launch(void *handle, void *stream) {
+  Batch *b = (Batch *)handle;
+  for (const Group &g : b->groups) {
+    const int rc = bfk_beamform_classic_batch(g.prec, g.tavg, g.nkc, g.nb, g.d_jobs, g.d_tile_start,
+                                              g.njobs, g.ntiles, stream);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+extern "C" int bfk_batch_launches(void *handle) {
+  return handle ? (int)((Batch *)handle)->groups.size() : 0;
+}

@@ -31,14 +31,24 @@ typedef struct bfk_dims {
   int prec;       /* 0 FP16, 1 FP8 hi+lo                                  */
   int order;      /* 0 capture, 1 split -r                                */
   int stuff;      /* 0 none, 1 antennas, 2 incoherent beam 0              */
-  int sched;      /* fused-kernel schedule, BFK_SCHED_*                   */
+  int variant;    /* fused-kernel variant, BFK_VAR_*                      */
 } bfk_dims;
 
-/* Schedules of the fused beamformer (same results, different speed):
- * CLASSIC   one tile per workgroup, two workgroups per CU;
- * PIPELINED one persistent workgroup per CU, staging and weight loads
- *           overlapped with the MFMA work in software. */
-enum { BFK_SCHED_CLASSIC = 0, BFK_SCHED_PIPELINED = 1, BFK_NSCHED = 2 };
+/* Variants of the fused beamformer (same bits, different speed):
+ * CLASSIC_*    one tile per workgroup, two workgroups per CU; 256, 128 or 64
+ *              beams per tile (128S: 2 beam tiles per wave, wave pairs split a
+ *              tile's samples; 128: one beam tile per wave; 64: one beam tile
+ *              per wave, split);
+ * PIPELINED_256 one persistent workgroup per CU, staging and weight loads
+ *              overlapped with the MFMA work in software (nant <= 64). */
+enum {
+  BFK_VAR_CLASSIC_256 = 0,
+  BFK_VAR_PIPELINED_256 = 1,
+  BFK_VAR_CLASSIC_128S = 2,
+  BFK_VAR_CLASSIC_128 = 3,
+  BFK_VAR_CLASSIC_64 = 4,
+  BFK_NVAR = 5
+};
 
 /* Bytes of the weight fragment buffer and of the per-(channel,pol,beam)
  * weight scale buffer for dims. */
@@ -50,8 +60,8 @@ int bfk_weights(const bfk_dims *d, const float *d_antpos, const float *d_cal,
                 const double *d_freqs, const double *d_theta, void *d_wfrag,
                 float *d_wscale, void *stream);
 
-/* Per-tile beam power sums written by every beamforming launch:
- * [nbeam][nsamp/64][ngroup] floats. */
+/* Per-32-sample beam power sums written by every beamforming launch:
+ * [nbeam][nsamp/32][ngroup] floats. */
 size_t bfk_bpart_bytes(const bfk_dims *d);
 
 /* Fused unpack -> MFMA beam GEMM -> |B|^2 -> average -> requantise.
@@ -60,8 +70,27 @@ int bfk_beamform(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
                  const float *d_wscale, const float *d_scale, uint8_t *d_out,
                  float *d_power, float *d_bpart, void *stream);
 
-/* 1 if schedule sched (BFK_SCHED_*) can run dims d, else 0. */
-int bfk_sched_supported(const bfk_dims *d, int sched);
+/* 1 if variant v (BFK_VAR_*) can run dims d, else 0. */
+int bfk_variant_supported(const bfk_dims *d, int v);
+
+/* Batched classic launches: all blocks of a batch in one launch per kernel
+ * instantiation (precision, tavg, antenna chunks, beam tiles per wave).
+ * prepare builds the device job tables for n blocks (dims[i] with its own
+ * variant; buffers as for bfk_beamform); launch enqueues the launches (no
+ * allocation, no host synchronisation: capturable in a graph). */
+typedef struct bfk_batch_bufs {
+  const uint8_t *d_in;
+  const void *d_wfrag;
+  const float *d_wscale;
+  const float *d_scale;
+  uint8_t *d_out;
+  float *d_power;       /* optional */
+  float *d_bpart;
+} bfk_batch_bufs;
+int bfk_batch_prepare(const bfk_dims *dims, const bfk_batch_bufs *bufs, int n, void **handle);
+int bfk_batch_launch(void *handle, void *stream);
+int bfk_batch_launches(void *handle);   /* kernel launches per bfk_batch_launch */
+void bfk_batch_free(void *handle);
 
 /* Antenna-stuffing (-s) and incoherent-beam (-q) outputs (rewrites the
  * d_bpart entries of the beams it writes). */

@@ -58,6 +58,7 @@ static void fill_dims(const dsaxbf_config *c, bfk_dims *d) {
   d->prec = c->precision;
   d->order = c->input_order;
   d->stuff = c->stuffants;
+  d->variant = d->nkc == 1 ? BFK_VAR_CLASSIC_256 : BFK_VAR_CLASSIC_128;
 }
 
 void dsaxbf_destroy(dsaxbf_plan *p) {
@@ -241,30 +242,35 @@ int dsaxbf_sync(void *stream) {
   return DSAXBF_OK;
 }
 
-int dsaxbf_set_schedule(dsaxbf_plan *p, int sched) {
+int dsaxbf_set_variant(dsaxbf_plan *p, int v) {
   if (!p) return dsaxbf_fail(DSAXBF_EINVAL, "NULL plan");
-  if (sched < 0 || sched >= DSAXBF_NSCHED)
-    return dsaxbf_fail(DSAXBF_EINVAL, "schedule %d out of range", sched);
-  if (!bfk_sched_supported(&p->dims, sched))
-    return dsaxbf_fail(DSAXBF_EINVAL, "schedule %d cannot run this shape (nant %d, nbeam %d)", sched,
-                       p->cfg.nant, p->cfg.nbeam);
-  p->dims.sched = sched;
+  if (v < 0 || v >= DSAXBF_NVARIANT) return dsaxbf_fail(DSAXBF_EINVAL, "variant %d out of range", v);
+  if (!bfk_variant_supported(&p->dims, v))
+    return dsaxbf_fail(DSAXBF_EINVAL, "variant %d cannot run this shape (nant %d)", v, p->cfg.nant);
+  p->dims.variant = v;
   return DSAXBF_OK;
 }
 
-int dsaxbf_get_schedule(const dsaxbf_plan *p) { return p ? p->dims.sched : -1; }
+int dsaxbf_get_variant(const dsaxbf_plan *p) { return p ? p->dims.variant : -1; }
 
-int dsaxbf_autotune(dsaxbf_plan *p, int reps, float *ms) {
-  dsaxbf_clear_error();
+int dsaxbf_plan_internals(dsaxbf_plan *p, bfk_dims *dims, void **d_wfrag, float **d_wscale,
+                          float **d_bpart) {
   if (!p) return dsaxbf_fail(DSAXBF_EINVAL, "NULL plan");
+  if (!p->have_weights) return dsaxbf_fail(DSAXBF_ESTATE, "plan has no weights");
+  if (dims) *dims = p->dims;
+  if (d_wfrag) *d_wfrag = p->d_wfrag;
+  if (d_wscale) *d_wscale = p->d_wscale;
+  if (d_bpart) *d_bpart = p->d_bpart;
+  return DSAXBF_OK;
+}
+
+int dsaxbf_time_variants(dsaxbf_plan *p, const uint8_t *d_in, const float *d_scale, uint8_t *d_out,
+                         int reps, int classic_only, float *ms, int *best_out) {
+  if (!p || !d_in || !d_scale || !d_out) return dsaxbf_fail(DSAXBF_EINVAL, "NULL argument");
   if (reps < 1) return dsaxbf_fail(DSAXBF_EINVAL, "reps must be >= 1");
   if (!p->have_weights && p->dims.stuff != DSAXBF_STUFF_ANTS)
     return dsaxbf_fail(DSAXBF_ESTATE, "autotune before dsaxbf_set_weights");
   HIPCHK(hipSetDevice(p->cfg.device), "hipSetDevice");
-  const size_t nin = dsaxbf_input_bytes(&p->cfg);
-  /* any voltages do: the work does not depend on the values */
-  HIPCHK(hipMemsetAsync(p->d_in, 0x5a, nin, p->stream), "hipMemsetAsync");
-  HIPCHK(hipMemsetAsync(p->d_scale, 0, sizeof(float) * p->cfg.nbeam, p->stream), "hipMemsetAsync");
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0), "hipEventCreate");
   hipError_t he = hipEventCreate(&e1);
@@ -272,19 +278,19 @@ int dsaxbf_autotune(dsaxbf_plan *p, int reps, float *ms) {
     hipEventDestroy(e0);
     return dsaxbf_fail(DSAXBF_EHIP, "hipEventCreate: %s", hipGetErrorString(he));
   }
-  const int keep = p->dims.sched;
+  const int keep = p->dims.variant;
   int best = keep, rc = DSAXBF_OK;
   float best_ms = -1.f;
-  for (int s = 0; s < DSAXBF_NSCHED && rc == DSAXBF_OK; s++) {
-    if (!bfk_sched_supported(&p->dims, s)) {   /* not a candidate for this shape */
-      if (ms) ms[s] = -1.f;
+  for (int v = 0; v < DSAXBF_NVARIANT && rc == DSAXBF_OK; v++) {
+    if (!bfk_variant_supported(&p->dims, v) || (classic_only && v == BFK_VAR_PIPELINED_256)) {
+      if (ms) ms[v] = -1.f;   /* not a candidate for this shape */
       continue;
     }
-    p->dims.sched = s;
+    p->dims.variant = v;
     float t = 0.f;
     for (int r = -1; r < reps && rc == DSAXBF_OK; r++) {
       if (r == 0) he = hipEventRecord(e0, p->stream);
-      rc = dsaxbf_execute(p, p->d_in, p->d_scale, p->d_out, NULL, NULL, p->stream);
+      rc = dsaxbf_execute(p, d_in, d_scale, d_out, NULL, NULL, p->stream);
     }
     if (rc) break;
     he = hipEventRecord(e1, p->stream);
@@ -295,14 +301,28 @@ int dsaxbf_autotune(dsaxbf_plan *p, int reps, float *ms) {
       break;
     }
     t /= (float)reps;
-    if (ms) ms[s] = t;
+    if (ms) ms[v] = t;
     if (best_ms < 0.f || t < best_ms) {
       best_ms = t;
-      best = s;
+      best = v;
     }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  p->dims.sched = rc == DSAXBF_OK ? best : keep;
+  p->dims.variant = keep;
+  if (best_out) *best_out = best;
+  return rc;
+}
+
+int dsaxbf_autotune(dsaxbf_plan *p, int reps, float *ms) {
+  dsaxbf_clear_error();
+  if (!p) return dsaxbf_fail(DSAXBF_EINVAL, "NULL plan");
+  HIPCHK(hipSetDevice(p->cfg.device), "hipSetDevice");
+  /* any voltages do: the work does not depend on the values */
+  HIPCHK(hipMemsetAsync(p->d_in, 0x5a, dsaxbf_input_bytes(&p->cfg), p->stream), "hipMemsetAsync");
+  HIPCHK(hipMemsetAsync(p->d_scale, 0, sizeof(float) * p->cfg.nbeam, p->stream), "hipMemsetAsync");
+  int best;
+  int rc = dsaxbf_time_variants(p, p->d_in, p->d_scale, p->d_out, reps, 0, ms, &best);
+  if (rc == DSAXBF_OK) p->dims.variant = best;
   return rc;
 }

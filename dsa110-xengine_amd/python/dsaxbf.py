@@ -32,10 +32,13 @@ STUFF_ANTS = 1
 STUFF_IB = 2
 ORDER_CAPTURE = 0
 ORDER_SPLIT_R = 1
-SCHED_CLASSIC = 0
-SCHED_PIPELINED = 1
-NSCHED = 2
-SCHED_NAMES = {SCHED_CLASSIC: "classic", SCHED_PIPELINED: "pipelined"}
+VARIANT_CLASSIC_256 = 0
+VARIANT_PIPELINED_256 = 1
+VARIANT_CLASSIC_128S = 2
+VARIANT_CLASSIC_128 = 3
+VARIANT_CLASSIC_64 = 4
+NVARIANT = 5
+VARIANT_NAMES = {0: "classic256", 1: "pipelined256", 2: "classic128s", 3: "classic128", 4: "classic64"}
 
 ERRORS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "EIO", -5: "ESTATE", -6: "ENODEV"}
 
@@ -95,9 +98,14 @@ def _load():
         "dsaxbf_execute": (C.c_int, [P, P, P, P, P, P, P]),
         "dsaxbf_process_block": (C.c_int, [P, P, P, P, P, P]),
         "dsaxbf_beam_power": (C.c_int, [P, P, P]),
-        "dsaxbf_set_schedule": (C.c_int, [P, C.c_int]),
-        "dsaxbf_get_schedule": (C.c_int, [P]),
+        "dsaxbf_set_variant": (C.c_int, [P, C.c_int]),
+        "dsaxbf_get_variant": (C.c_int, [P]),
         "dsaxbf_autotune": (C.c_int, [P, C.c_int, P]),
+        "dsaxbf_batch_create": (C.c_int, [C.POINTER(P), P, C.c_int, P, P, P, P]),
+        "dsaxbf_batch_autotune": (C.c_int, [P, C.c_int, P]),
+        "dsaxbf_batch_launch": (C.c_int, [P, P]),
+        "dsaxbf_batch_kernels": (C.c_int, [P]),
+        "dsaxbf_batch_destroy": (None, [P]),
         "dsaxbf_pipe_create": (C.c_int, [C.POINTER(P), P]),
         "dsaxbf_pipe_destroy": (None, [P]),
         "dsaxbf_pipe_input": (P, [P, C.c_int]),
@@ -133,8 +141,9 @@ EXPORTED = [
     "dsaxbf_config_default", "dsaxbf_input_bytes", "dsaxbf_output_bytes", "dsaxbf_last_error",
     "dsaxbf_create", "dsaxbf_destroy", "dsaxbf_get_config", "dsaxbf_get_stream",
     "dsaxbf_read_weights_file", "dsaxbf_set_weights", "dsaxbf_reference_freqs",
-    "dsaxbf_execute", "dsaxbf_beam_power", "dsaxbf_set_schedule", "dsaxbf_get_schedule",
-    "dsaxbf_autotune", "dsaxbf_pipe_create", "dsaxbf_pipe_destroy", "dsaxbf_pipe_input",
+    "dsaxbf_execute", "dsaxbf_beam_power", "dsaxbf_set_variant", "dsaxbf_get_variant",
+    "dsaxbf_autotune", "dsaxbf_batch_create", "dsaxbf_batch_autotune", "dsaxbf_batch_launch",
+    "dsaxbf_batch_kernels", "dsaxbf_batch_destroy", "dsaxbf_pipe_create", "dsaxbf_pipe_destroy", "dsaxbf_pipe_input",
     "dsaxbf_pipe_upload", "dsaxbf_pipe_run", "dsaxbf_process_block", "dsaxbf_agc_init", "dsaxbf_agc_free",
     "dsaxbf_agc_update", "dsaxbf_requant", "dsaxbf_synth_block", "dsaxbf_synth_geometry",
     "dsaxbf_dev_alloc", "dsaxbf_dev_free", "dsaxbf_copy_h2d", "dsaxbf_copy_d2h", "dsaxbf_sync",
@@ -290,17 +299,17 @@ class Plan:
         _check(lib.dsaxbf_beam_power(self._h, p, stream))
 
     @property
-    def schedule(self) -> int:
-        return int(lib.dsaxbf_get_schedule(self._h))
+    def variant(self) -> int:
+        return int(lib.dsaxbf_get_variant(self._h))
 
-    @schedule.setter
-    def schedule(self, sched: int):
-        _check(lib.dsaxbf_set_schedule(self._h, sched))
+    @variant.setter
+    def variant(self, v: int):
+        _check(lib.dsaxbf_set_variant(self._h, v))
 
     def autotune(self, reps: int = 5) -> np.ndarray:
-        """Time every schedule on this plan's shape, keep the fastest; returns
-        the mean launch time (ms) of each."""
-        ms = np.zeros(NSCHED, np.float32)
+        """Time every variant this plan's shape supports, keep the fastest;
+        returns the mean launch time (ms) of each (-1 = cannot run)."""
+        ms = np.zeros(NVARIANT, np.float32)
         _check(lib.dsaxbf_autotune(self._h, reps, _ptr(ms)))
         return ms
 
@@ -347,6 +356,49 @@ class Pipe:
     def close(self):
         if self._h:
             lib.dsaxbf_pipe_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Blocks of different shapes bound to device buffers, replayed as one HIP
+    graph (dsaxbf_batch_*).  plans and buffers must outlive the batch."""
+
+    def __init__(self, plans, d_in, d_scale, d_out, d_beam_power=None):
+        n = len(plans)
+        P = C.c_void_p
+
+        def arr(bufs):
+            return (P * n)(*[None if b is None else b.ptr for b in bufs])
+        self.plans = list(plans)
+        hs = (P * n)(*[p._h for p in plans])
+        self._arrays = (hs, arr(d_in), arr(d_scale), arr(d_out),
+                        None if d_beam_power is None else arr(d_beam_power))
+        self._h = P()
+        _check(lib.dsaxbf_batch_create(C.byref(self._h), hs, n, self._arrays[1], self._arrays[2],
+                                       self._arrays[3], self._arrays[4]))
+        self.n = n
+
+    def autotune(self, reps: int = 3) -> np.ndarray:
+        ms = np.zeros((self.n, NVARIANT), np.float32)
+        _check(lib.dsaxbf_batch_autotune(self._h, reps, _ptr(ms)))
+        return ms
+
+    def launch(self, stream=None):
+        _check(lib.dsaxbf_batch_launch(self._h, stream))
+
+    @property
+    def kernels(self) -> int:
+        return int(lib.dsaxbf_batch_kernels(self._h))
+
+    def close(self):
+        if self._h:
+            lib.dsaxbf_batch_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

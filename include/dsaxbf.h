@@ -176,26 +176,39 @@ int dsaxbf_execute(dsaxbf_plan *plan, const uint8_t *d_in, const float *d_scale,
  * itself when given d_beam_power.  Same stream as that execute. */
 int dsaxbf_beam_power(dsaxbf_plan *plan, float *d_beam_power, void *stream);
 
-/* Schedules of the fused kernel.  All give bit-identical results; which is
- * fastest depends on the shape and the GPU.
- *   CLASSIC    one tile per workgroup, two workgroups per CU;
- *   PIPELINED  one persistent workgroup per CU with the next tile's staging
- *              and the next weights' loads overlapped with the MFMA work.
+/* Variants of the fused kernel: two schedules and three tile shapes.  All
+ * give bit-identical results; which is fastest depends on the shape (few beams
+ * favour small tiles) and the GPU.
+ *   CLASSIC_256    one tile (256 beams x 64 samples x 8 channels) per
+ *                  workgroup, two workgroups per CU (default for nant <= 64);
+ *   PIPELINED_256  one persistent workgroup per CU with the next tile's
+ *                  staging and the next weights' loads overlapped with the MFMA
+ *                  work (nant <= 64);
+ *   CLASSIC_128S   128-beam tiles: 64 beams per wave, wave pairs split the
+ *                  tile's samples (nant <= 64);
+ *   CLASSIC_128    128-beam tiles: 32 beams per wave (default for nant > 64);
+ *   CLASSIC_64     64-beam tiles: 32 beams per wave, samples split.
  * No counterpart in the reference (its five kernels per int have one fixed
  * launch shape, dsaX_beamformer.cu:955-1003). */
-enum { DSAXBF_SCHED_CLASSIC = 0, DSAXBF_SCHED_PIPELINED = 1, DSAXBF_NSCHED = 2 };
+enum {
+  DSAXBF_VARIANT_CLASSIC_256 = 0,
+  DSAXBF_VARIANT_PIPELINED_256 = 1,
+  DSAXBF_VARIANT_CLASSIC_128S = 2,
+  DSAXBF_VARIANT_CLASSIC_128 = 3,
+  DSAXBF_VARIANT_CLASSIC_64 = 4,
+  DSAXBF_NVARIANT = 5
+};
 
-/* Select the schedule used by later executes on this plan (default CLASSIC);
- * DSAXBF_EINVAL if it cannot run the plan's shape. */
-int dsaxbf_set_schedule(dsaxbf_plan *plan, int sched);
-int dsaxbf_get_schedule(const dsaxbf_plan *plan);
+/* Select the variant used by later executes on this plan; DSAXBF_EINVAL if
+ * it cannot run the plan's shape. */
+int dsaxbf_set_variant(dsaxbf_plan *plan, int variant);
+int dsaxbf_get_variant(const dsaxbf_plan *plan);
 
-/* Autotuner: times every schedule on this plan's shape (the plan's own
- * device buffers, current weights; reps launches each after one warm-up) and
- * keeps the fastest.  ms, if not NULL, receives DSAXBF_NSCHED mean launch
- * times in milliseconds (-1 for a schedule that cannot run this shape: the
- * pipelined one needs nant <= 64).  Synchronous; requires dsaxbf_set_weights (or -s
- * mode).  No counterpart in the reference. */
+/* Autotuner: times every variant the shape supports (the plan's own device
+ * buffers, current weights; reps launches each after one warm-up) and keeps
+ * the fastest.  ms, if not NULL, receives DSAXBF_NVARIANT mean launch times in
+ * milliseconds (-1 for a variant that cannot run this shape).  Synchronous;
+ * requires dsaxbf_set_weights (or -s mode).  No counterpart in the reference. */
 int dsaxbf_autotune(dsaxbf_plan *plan, int reps, float *ms);
 
 /* Same as dsaxbf_execute with host pointers: copies the block in, runs it,
@@ -255,6 +268,32 @@ int dsaxbf_synth_geometry(int nant, int ncal, int nbeam, uint64_t seed,
                           double length_m, double amp_jitter,
                           double half_span_deg, float *antpos, float *cal,
                           double *theta_rad);
+
+
+/* ---- batches: many blocks of different shapes in one graph launch ------- */
+/* A batch binds n plans (each with weights set; no -s/-q mode) to device
+ * buffers: d_in[i] (input_bytes of plan i), d_scale[i] (nbeam floats),
+ * d_out[i] (output_bytes), d_beam_power[i] (nbeam floats, or NULL for none;
+ * the array itself may be NULL).  Blocks that share precision, tavg and
+ * kernel shape run in one launch over all their tiles; the launches (and the
+ * per-block beam-power sums) are captured once into a HIP graph, which
+ * dsaxbf_batch_launch replays.  The plans must outlive the batch; their
+ * variants at creation (or after dsaxbf_batch_autotune) fix the tile shapes.
+ * No counterpart in the reference (one fixed-shape block per process). */
+typedef struct dsaxbf_batch dsaxbf_batch;
+int dsaxbf_batch_create(dsaxbf_batch **batch, dsaxbf_plan *const *plans, int n,
+                        const uint8_t *const *d_in, const float *const *d_scale,
+                        uint8_t *const *d_out, float *const *d_beam_power);
+/* Autotuner for a batch: for each distinct block shape, time every classic
+ * variant on that block's buffers, set the fastest on the plans of that
+ * shape, then rebuild the graph.  ms, if not NULL, receives n*DSAXBF_NVARIANT
+ * times (row i for block i, -1 where not run). */
+int dsaxbf_batch_autotune(dsaxbf_batch *batch, int reps, float *ms);
+/* Replay the batch graph on stream (NULL = the batch's own stream). */
+int dsaxbf_batch_launch(dsaxbf_batch *batch, void *stream);
+/* Kernel launches in one replay (beamforming groups + beam-power sums). */
+int dsaxbf_batch_kernels(const dsaxbf_batch *batch);
+void dsaxbf_batch_destroy(dsaxbf_batch *batch);
 
 /* ---- streaming (replaces the per-int copy/launch loop on 4 streams with
  *      pageable copies, dsaX_beamformer.cu:955-1003) -------------------- */

@@ -136,11 +136,11 @@ def test_small_block_golden(bf, orc, prec):
     assert (du == 1).mean() <= max(4 * expect, 1e-3)
 
 
-def _run_sched(bf, cfg, block, geo, scale, sched):
+def _run_variant(bf, cfg, block, geo, scale, var):
     plan = bf.Plan(cfg)
     plan.set_weights(*geo)
-    plan.schedule = sched
-    assert plan.schedule == sched
+    plan.variant = var
+    assert plan.variant == var
     res = plan.process_block(block, scale, want_power=True, want_beam_power=True)
     plan.close()
     return res
@@ -153,10 +153,14 @@ def _run_sched(bf, cfg, block, geo, scale, sched):
     (63, 16, 128, 512, 0, 64),
     (64, 64, 2048, 512, 0, 16),    # 1024 tiles: several per persistent workgroup
     (24, 64, 2048, 256, 1, 32),
+    (48, 32, 256, 64, 1, 64),      # 64 beams, split TAVG-64 bins through LDS
+    (64, 16, 128, 130, 0, 8),
+    (100, 16, 128, 64, 0, 64),     # two antenna chunks, 64 beams
+    (256, 16, 128, 200, 0, 16),
 ])
-def test_schedules_bit_identical(bf, orc, prec, nant, nchan, npkt, nbeam, order, tavg):
-    """Every fused-kernel schedule gives the same bits (same MFMA chains and
-    the same detect order), and the pipelined one matches the oracle."""
+def test_variants_bit_identical(bf, orc, prec, nant, nchan, npkt, nbeam, order, tavg):
+    """Every fused-kernel variant the shape supports gives the same bits (same
+    MFMA chains and the same detect order), and they match the oracle."""
     cfg = bf.config(nant=nant, nchan=nchan, npkt=npkt, nbeam=nbeam, tavg=tavg, precision=prec,
                     weight_mode=1, angle_mode=1, input_order=order)
     antpos, cal, theta = bf.synth_geometry(nant, cfg.ncal, nbeam, nant + nchan, 1200.0, 0.1, 1.75)
@@ -164,10 +168,11 @@ def test_schedules_bit_identical(bf, orc, prec, nant, nchan, npkt, nbeam, order,
     block = bf.synth_block(cfg, nbeam + tavg)
     geo = (antpos, cal, freqs, theta)
     scale = np.full(nbeam, 2.0 ** -12, np.float32)
-    ref = [_run_sched(bf, cfg, block, geo, scale, s) for s in range(bf.NSCHED)]
-    for s in range(1, bf.NSCHED):
-        for a, b in zip(ref[0], ref[s]):
-            assert np.array_equal(a, b), bf.SCHED_NAMES[s]
+    variants = [v for v in range(bf.NVARIANT) if nant <= 64 or v not in (0, 1, 2)]
+    ref = [_run_variant(bf, cfg, block, geo, scale, v) for v in variants]
+    for k in range(1, len(variants)):
+        for a, b in zip(ref[0], ref[k]):
+            assert np.array_equal(a, b), bf.VARIANT_NAMES[variants[k]]
     d = orc.dims(nant, nchan, npkt, nbeam, tavg, order)
     W = orc.weights(d, 1, 1, antpos, cal, freqs, theta)
     g = (0, 2)
@@ -177,7 +182,7 @@ def test_schedules_bit_identical(bf, orc, prec, nant, nchan, npkt, nbeam, order,
     _check(r, prec)
 
 
-def test_autotune_picks_a_schedule(bf):
+def test_autotune_picks_a_variant(bf):
     cfg = bf.config(nant=64, nchan=64, npkt=512, nbeam=256, tavg=32, weight_mode=1, angle_mode=1)
     antpos, cal, theta = bf.synth_geometry(64, 64, 256, 4)
     plan = bf.Plan(cfg)
@@ -185,11 +190,16 @@ def test_autotune_picks_a_schedule(bf):
         plan.autotune(2)                     # no weights yet
     plan.set_weights(antpos, cal, bf.reference_freqs(1498.75, 64), theta)
     ms = plan.autotune(3)
-    assert ms.shape == (bf.NSCHED,) and (ms > 0).all()
-    assert plan.schedule == int(np.argmin(ms))
+    assert ms.shape == (bf.NVARIANT,) and (ms > 0).all()
+    assert plan.variant == int(np.argmin(ms))
     with pytest.raises(bf.BeamformerError):
-        plan.schedule = bf.NSCHED
+        plan.variant = bf.NVARIANT
     plan.close()
+    big = bf.Plan(bf.config(nant=100, nchan=16, npkt=128, nbeam=256))
+    with pytest.raises(bf.BeamformerError):
+        big.variant = bf.VARIANT_PIPELINED_256          # needs nant <= 64
+    assert big.variant == bf.VARIANT_CLASSIC_128
+    big.close()
 
 
 def test_beam_power_matches_oracle(bf, orc):
@@ -372,3 +382,61 @@ def test_pipe_streams_like_process_block(bf):
         pipe.run(0, scale)                     # slot 0 not uploaded again
     pipe.close()
     plan.close()
+
+
+def test_batch_graph_matches_per_block(bf):
+    """Blocks of different shapes and precisions in one batch graph: every
+    replay (with eager work and synchronisation between replays) equals the
+    per-block executes, before and after the batch autotuner re-picks the tile
+    shapes."""
+    shapes = [  # nant, nchan, npkt, nbeam, tavg, prec
+        (64, 16, 128, 64, 8, 0), (64, 48, 64, 128, 32, 0), (40, 24, 256, 256, 32, 1),
+        (64, 8, 128, 512, 16, 1), (100, 16, 64, 100, 32, 0), (64, 16, 128, 64, 8, 0)]
+    plans, bufs, want = [], [], []
+    for i, (na, nc, npk, nb, ta, pr) in enumerate(shapes):
+        cfg = bf.config(nant=na, nchan=nc, npkt=npk, nbeam=nb, tavg=ta, precision=pr,
+                        weight_mode=1, angle_mode=1)
+        antpos, cal, theta = bf.synth_geometry(na, cfg.ncal, nb, 30 + i)
+        plan = bf.Plan(cfg)
+        plan.set_weights(antpos, cal, bf.reference_freqs(1498.75, nc), theta)
+        blk = bf.synth_block(cfg, 90 + i)
+        d_in, d_sc = bf.DeviceBuffer(blk.nbytes), bf.DeviceBuffer(4 * nb)
+        d_out, d_bp = bf.DeviceBuffer(bf.output_bytes(cfg)), bf.DeviceBuffer(4 * nb)
+        d_in.upload(blk)
+        d_sc.upload(np.full(nb, 2.0 ** -10, np.float32))
+        plan.execute(d_in, d_sc, d_out, None, d_bp)
+        want.append((d_out.download(np.empty(bf.output_bytes(cfg), np.uint8), plan.stream),
+                     d_bp.download(np.empty(nb, np.float32), plan.stream)))
+        plans.append(plan)
+        bufs.append((d_in, d_sc, d_out, d_bp, cfg))
+
+    def check(tag):
+        for i, (d_in, d_sc, d_out, d_bp, cfg) in enumerate(bufs):
+            out = d_out.download(np.empty(bf.output_bytes(cfg), np.uint8))
+            bp = d_bp.download(np.empty(cfg.nbeam, np.float32))
+            assert np.array_equal(out, want[i][0]), (tag, i)
+            assert np.array_equal(bp, want[i][1]), (tag, i)
+
+    def clear():
+        for d_in, d_sc, d_out, d_bp, cfg in bufs:
+            d_out.upload(np.zeros(bf.output_bytes(cfg), np.uint8))
+            d_bp.upload(np.zeros(cfg.nbeam, np.float32))
+
+    batch = bf.Batch(plans, [b[0] for b in bufs], [b[1] for b in bufs], [b[2] for b in bufs],
+                     [b[3] for b in bufs])
+    assert batch.kernels >= 3 + len(shapes)    # >= 3 kernel instantiations + beam-power sums
+    for rep in range(3):
+        clear()
+        batch.launch()
+        check(f"replay {rep}")
+        plans[0].execute(bufs[0][0], bufs[0][1], bufs[0][2], None, bufs[0][3])   # eager work between
+    ms = batch.autotune(2)
+    assert ms.shape == (len(shapes), bf.NVARIANT)
+    assert (ms[:, bf.VARIANT_PIPELINED_256] < 0).all()   # batches run classic tiles only
+    for rep in range(2):
+        clear()
+        batch.launch()
+        check(f"tuned replay {rep}")
+    batch.close()
+    for p in plans:
+        p.close()
