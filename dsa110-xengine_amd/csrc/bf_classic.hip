@@ -42,7 +42,7 @@ template <int NKC> struct Shape {
 }
 
 // One tile L of job a (all threads of the workgroup).
-template <int PREC, int TAVG, int NKC, int NB>
+template <int PREC, int TAVG, int NKC, int NB, int SPLIT>
 __device__ __forceinline__ void classic_tile(const BfArgs &a, int L, uint8_t *stage) {
   using S = Shape<NKC>;
   constexpr int ROWB_ = S::ROWB, NCI = S::NCI, CQS = S::CQS;
@@ -51,11 +51,11 @@ __device__ __forceinline__ void classic_tile(const BfArgs &a, int L, uint8_t *st
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int n = lane & 31, h = lane >> 5;
-  const int split = a.split;
-  const int nbq = NW >> split;                           // beam groups per tile
+  constexpr int nbq = NW >> SPLIT;                       // beam groups per tile
+  constexpr int ntt = SPLIT ? 1 : 2;                     // units per wave
   const int bq = wave % nbq, th = wave / nbq;            // th: unit of a split pair
   const int btw = (t.bset * nbq + bq) * NB;              // first beam tile of this wave
-  const int tt_first = split ? th : 0, ntt = split ? 1 : 2;
+  const int tt_first = SPLIT ? th : 0;
 
   float part[2][NB][NP];             // per unit this wave runs
 #pragma unroll
@@ -104,19 +104,17 @@ __device__ __forceinline__ void classic_tile(const BfArgs &a, int L, uint8_t *st
       WFrag<PREC, NB, NKC> W;
       W.load(a.wfrag, a.wscale, wr * a.nbt + btw, wr * a.nbeam_pad + btw * 32, lane, n);
 #pragma unroll
-      for (int k = 0; k < 2; k++) {
-        if (k < ntt) {
-          Acc<NB> A;
-          gemm_unit(A, W, stage + (cql * TCU + (tt_first + k) * 32) * ROWB_, ROWB_, n, h);
-          detect<TAVG>(A, W.sc, part[k]);
-        }
+      for (int k = 0; k < ntt; k++) {
+        Acc<NB> A;
+        gemm_unit(A, W, stage + (cql * TCU + (tt_first + k) * 32) * ROWB_, ROWB_, n, h);
+        detect<TAVG>(A, W.sc, part[k]);
       }
     }
   }
 
   // (c) a split TAVG-64 bin needs both units: the second wave of each pair
   // hands its half to the first through LDS
-  if (TAVG == 64 && split) {
+  if (TAVG == 64 && SPLIT) {
     float *xch = (float *)stage;
     __syncthreads();                   // every wave is done with the stage
     if (th == 1)
@@ -133,7 +131,7 @@ __device__ __forceinline__ void classic_tile(const BfArgs &a, int L, uint8_t *st
   tile_output<TAVG>(a, t, btw, n, h, part, tt_first, ntt);
 }
 
-template <int PREC, int TAVG, int NKC, int NB>
+template <int PREC, int TAVG, int NKC, int NB, int SPLIT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void bf_classic_kernel(BfArgs a) {
   __shared__ __attribute__((aligned(16))) uint8_t stage[Shape<NKC>::STAGE];
@@ -142,12 +140,12 @@ void bf_classic_kernel(BfArgs a) {
   const int P = blockIdx.x;
   const int L = (P & 7) * (gridDim.x >> 3) + (P >> 3);
   if (L >= a.ntiles) return;
-  classic_tile<PREC, TAVG, NKC, NB>(a, L, stage);
+  classic_tile<PREC, TAVG, NKC, NB, SPLIT>(a, L, stage);
 }
 
 // Batched: the tiles of njobs blocks back to back; job j owns global tiles
 // [tile_start[j], tile_start[j+1]).
-template <int PREC, int TAVG, int NKC, int NB>
+template <int PREC, int TAVG, int NKC, int NB, int SPLIT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void bf_classic_batch_kernel(const BfArgs *__restrict__ jobs, const int *__restrict__ tile_start,
                              int njobs) {
@@ -157,7 +155,7 @@ void bf_classic_batch_kernel(const BfArgs *__restrict__ jobs, const int *__restr
   if (L >= tile_start[njobs]) return;
   int j = 0;
   while (L >= tile_start[j + 1]) j++;
-  classic_tile<PREC, TAVG, NKC, NB>(jobs[j], L - tile_start[j], stage);
+  classic_tile<PREC, TAVG, NKC, NB, SPLIT>(jobs[j], L - tile_start[j], stage);
 }
 
 // ---- host side --------------------------------------------------------------
@@ -179,14 +177,14 @@ extern "C" int bfk_classic_beams(int variant, int nkc) {
   return (NW >> split) * nb * 32;
 }
 
-template <int PREC, int NKC, int NB>
+template <int PREC, int NKC, int NB, int SPLIT>
 static void launch(int tavg, dim3 grid, hipStream_t s, const BfArgs *a, const BfArgs *jobs,
                    const int *tile_start, int njobs) {
-#define BF_LAUNCH(T)                                                                                   \
-  do {                                                                                                 \
-    if (a) hipLaunchKernelGGL((bf_classic_kernel<PREC, T, NKC, NB>), grid, dim3(NT), 0, s, *a);      \
-    else hipLaunchKernelGGL((bf_classic_batch_kernel<PREC, T, NKC, NB>), grid, dim3(NT), 0, s, jobs, \
-                            tile_start, njobs);                                                        \
+#define BF_LAUNCH(T)                                                                                  \
+  do {                                                                                                \
+    if (a) hipLaunchKernelGGL((bf_classic_kernel<PREC, T, NKC, NB, SPLIT>), grid, dim3(NT), 0, s, *a); \
+    else hipLaunchKernelGGL((bf_classic_batch_kernel<PREC, T, NKC, NB, SPLIT>), grid, dim3(NT), 0, s,  \
+                            jobs, tile_start, njobs);                                                 \
   } while (0)
   switch (tavg) {
     case 8: BF_LAUNCH(8); break;
@@ -199,22 +197,23 @@ static void launch(int tavg, dim3 grid, hipStream_t s, const BfArgs *a, const Bf
 
 // Launch either one job (a != NULL) or a batch (device tables) of one
 // instantiation (prec, tavg, nkc, nb).
-static int dispatch(int prec, int tavg, int nkc, int nb, dim3 grid, hipStream_t s, const BfArgs *a,
-                    const BfArgs *jobs, const int *tile_start, int njobs) {
-  const int key = (nkc << 2) | (nb << 1) | prec;
+static int dispatch(int prec, int tavg, int nkc, int nb, int split, dim3 grid, hipStream_t s,
+                    const BfArgs *a, const BfArgs *jobs, const int *tile_start, int njobs) {
+  const int key = (nkc << 3) | (nb << 2) | (split << 1) | prec;
+#define BF_CASE(P, K, B, S)                                                     \
+  case (K << 3) | (B << 2) | (S << 1) | P:                                      \
+    launch<P, K, B, S>(tavg, grid, s, a, jobs, tile_start, njobs);              \
+    break
   switch (key) {
-    case (1 << 2) | (2 << 1) | 0: launch<0, 1, 2>(tavg, grid, s, a, jobs, tile_start, njobs); break;
-    case (1 << 2) | (2 << 1) | 1: launch<1, 1, 2>(tavg, grid, s, a, jobs, tile_start, njobs); break;
-    case (1 << 2) | (1 << 1) | 0: launch<0, 1, 1>(tavg, grid, s, a, jobs, tile_start, njobs); break;
-    case (1 << 2) | (1 << 1) | 1: launch<1, 1, 1>(tavg, grid, s, a, jobs, tile_start, njobs); break;
-    case (2 << 2) | (1 << 1) | 0: launch<0, 2, 1>(tavg, grid, s, a, jobs, tile_start, njobs); break;
-    case (2 << 2) | (1 << 1) | 1: launch<1, 2, 1>(tavg, grid, s, a, jobs, tile_start, njobs); break;
-    case (4 << 2) | (1 << 1) | 0: launch<0, 4, 1>(tavg, grid, s, a, jobs, tile_start, njobs); break;
-    case (4 << 2) | (1 << 1) | 1: launch<1, 4, 1>(tavg, grid, s, a, jobs, tile_start, njobs); break;
+    BF_CASE(0, 1, 2, 0); BF_CASE(1, 1, 2, 0); BF_CASE(0, 1, 2, 1); BF_CASE(1, 1, 2, 1);
+    BF_CASE(0, 1, 1, 0); BF_CASE(1, 1, 1, 0); BF_CASE(0, 1, 1, 1); BF_CASE(1, 1, 1, 1);
+    BF_CASE(0, 2, 1, 0); BF_CASE(1, 2, 1, 0); BF_CASE(0, 2, 1, 1); BF_CASE(1, 2, 1, 1);
+    BF_CASE(0, 4, 1, 0); BF_CASE(1, 4, 1, 0); BF_CASE(0, 4, 1, 1); BF_CASE(1, 4, 1, 1);
     default:
       bfk_set_error("bfk_beamform: no classic kernel for %d antenna chunks x %d beam tiles", nkc, nb);
       return -1;
   }
+#undef BF_CASE
   return bfk_check(hipGetLastError(), "bf_classic_kernel");
 }
 
@@ -241,13 +240,13 @@ extern "C" int bfk_beamform_classic(const bfk_dims *d, const uint8_t *d_in, cons
   int nb;
   if (bfk_classic_args(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, &a, &nb)) return -1;
   const dim3 grid((a.ntiles + 7) / 8 * 8);
-  return dispatch(d->prec, d->tavg, d->nkc, nb, grid, (hipStream_t)stream, &a, NULL, NULL, 0);
+  return dispatch(d->prec, d->tavg, d->nkc, nb, a.split, grid, (hipStream_t)stream, &a, NULL, NULL, 0);
 }
 
-extern "C" int bfk_beamform_classic_batch(int prec, int tavg, int nkc, int nb, const void *d_jobs,
-                                          const int *d_tile_start, int njobs, int ntiles,
-                                          void *stream) {
+extern "C" int bfk_beamform_classic_batch(int prec, int tavg, int nkc, int nb, int split,
+                                          const void *d_jobs, const int *d_tile_start, int njobs,
+                                          int ntiles, void *stream) {
   const dim3 grid((ntiles + 7) / 8 * 8);
-  return dispatch(prec, tavg, nkc, nb, grid, (hipStream_t)stream, NULL, (const BfArgs *)d_jobs,
+  return dispatch(prec, tavg, nkc, nb, split, grid, (hipStream_t)stream, NULL, (const BfArgs *)d_jobs,
                   d_tile_start, njobs);
 }

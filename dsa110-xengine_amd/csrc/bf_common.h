@@ -279,6 +279,11 @@ template <int NB = BTW> struct Acc {
 template <int PREC, int NB, int NK>
 __device__ __forceinline__ void gemm_unit(Acc<NB> &A, const WFrag<PREC, NB, NK> &W, const uint8_t *rows,
                                           int rowb, int n, int h) {
+  // A wave issuing its MFMA chain runs at raised issue priority, so on a SIMD
+  // shared by two waves the matrix pipe is fed first and the other wave's
+  // detect/staging VALU work fills the gaps (measured: 8 % faster FP16 block,
+  // and the scheduling boundary keeps register use at ~170).
+  __builtin_amdgcn_s_setprio(1);
 #pragma unroll
   for (int bt = 0; bt < NB; bt++) { A.r[bt] = (f16v){}; A.i[bt] = (f16v){}; }
 #pragma unroll
@@ -310,6 +315,8 @@ __device__ __forceinline__ void gemm_unit(Acc<NB> &A, const WFrag<PREC, NB, NK> 
           aim[2 * u] = (int)di;
           aim[2 * u + 1] = (int)dr;
         }
+        // hi parts of every accumulator, then lo parts: 2*NB - 1 MFMAs sit
+        // between two that update the same accumulator
 #pragma unroll
         for (int bt = 0; bt < NB; bt++) {
           A.r[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.hi[bt][kc][s2], A.r[bt], 0, 0, 0, 0, 0, 0);
@@ -323,6 +330,7 @@ __device__ __forceinline__ void gemm_unit(Acc<NB> &A, const WFrag<PREC, NB, NK> 
       }
     }
   }
+  __builtin_amdgcn_s_setprio(0);
 }
 
 // |B|^2 summed over the accumulator rows of each bin (four independent FMA
