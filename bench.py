@@ -347,7 +347,7 @@ def main():
     ap.add_argument("--check-groups", type=int, default=8)
     ap.add_argument("--variant", default="auto",
                     choices=["auto", "classic256", "pipelined256", "classic128s", "classic128",
-                             "classic64"],
+                             "classic64", "shared256"],
                     help="fused-kernel variant (auto: the library's autotuner picks)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched uneven-blocks measurement (one graph launch)")
@@ -442,8 +442,8 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[main_p],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[main_p],
                      "traffic": traffic_from_profiles(main_p),
-                     "kernel": "bf_pipelined_kernel" if r["variant"] == "pipelined256"
-                     else "bf_classic_kernel", "variant": r["variant"], "kernel_ms": r["kernel_ms"],
+                     "kernel": {"pipelined256": "bf_pipelined_kernel",
+                                "shared256": "bf_shared_kernel"}.get(r["variant"], "bf_classic_kernel"), "variant": r["variant"], "kernel_ms": r["kernel_ms"],
                      "flop_per_launch": FLOP_PER_BLOCK},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
                                                   ("value", "unit", "cores", "kind", "sample")},

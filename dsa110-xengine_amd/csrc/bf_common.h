@@ -201,6 +201,18 @@ template <int NB, int NK> struct WFrag<0, NB, NK> {
 #pragma unroll
     for (int bt = 0; bt < NB; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
   }
+  // One K step of every beam tile (NK == 1), for a rolling prefetch that
+  // refills step s as soon as the last MFMA reading it has issued.
+  __device__ __forceinline__ void load_step(const uint8_t *wfrag, long wrow_bt, int s, int lane) {
+    static_assert(NK == 1, "rolling prefetch is for single-chunk tiles");
+#pragma unroll
+    for (int bt = 0; bt < NB; bt++)
+      w[bt][0][s] = *(const h8v *)(wfrag + (((wrow_bt + bt) * 8 + s) * 64 + lane) * 16);
+  }
+  __device__ __forceinline__ void load_scale(const float *wscale, long sc_off, int n) {
+#pragma unroll
+    for (int bt = 0; bt < NB; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
+  }
   __device__ __forceinline__ void load_buf(__amdgpu_buffer_rsrc_t rw, __amdgpu_buffer_rsrc_t rs,
                                            uint32_t wrow_bt, uint32_t sc_off, uint32_t vlane16,
                                            uint32_t vn4) {
@@ -235,6 +247,20 @@ template <int NB, int NK> struct WFrag<1, NB, NK> {
           hi[bt][kc][s2] = *(const i8v *)(wfrag + rec * 32);
           lo[bt][kc][s2] = *(const i8v *)(wfrag + (rec + 64) * 32);
         }
+#pragma unroll
+    for (int bt = 0; bt < NB; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
+  }
+  // Step s2 (hi and lo) of every beam tile (NK == 1); see WFrag<0>::load_step.
+  __device__ __forceinline__ void load_step(const uint8_t *wfrag, long wrow_bt, int s2, int lane) {
+    static_assert(NK == 1, "rolling prefetch is for single-chunk tiles");
+#pragma unroll
+    for (int bt = 0; bt < NB; bt++) {
+      const long rec = (((wrow_bt + bt) * 2 + s2) * 2) * 64 + lane;
+      hi[bt][0][s2] = *(const i8v *)(wfrag + rec * 32);
+      lo[bt][0][s2] = *(const i8v *)(wfrag + (rec + 64) * 32);
+    }
+  }
+  __device__ __forceinline__ void load_scale(const float *wscale, long sc_off, int n) {
 #pragma unroll
     for (int bt = 0; bt < NB; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
   }

@@ -40,14 +40,17 @@ typedef struct bfk_dims {
  *              tile's samples; 128: one beam tile per wave; 64: one beam tile
  *              per wave, split);
  * PIPELINED_256 one persistent workgroup per CU, staging and weight loads
- *              overlapped with the MFMA work in software (nant <= 64). */
+ *              overlapped with the MFMA work in software (nant <= 64);
+ * SHARED_256   8 waves per tile that unpack each (channel, pol) once into
+ *              LDS operands all waves read (nant <= 64). */
 enum {
   BFK_VAR_CLASSIC_256 = 0,
   BFK_VAR_PIPELINED_256 = 1,
   BFK_VAR_CLASSIC_128S = 2,
   BFK_VAR_CLASSIC_128 = 3,
   BFK_VAR_CLASSIC_64 = 4,
-  BFK_NVAR = 5
+  BFK_VAR_SHARED_256 = 5,
+  BFK_NVAR = 6
 };
 
 /* Bytes of the weight fragment buffer and of the per-(channel,pol,beam)

@@ -1,0 +1,276 @@
+// bf_shared.hip -- the "shared" schedule of the fused beamformer: the waves
+// of a tile unpack each (channel, pol) once, into LDS MFMA operands that all
+// of them read, instead of each wave unpacking the same voltages.
+//
+// Reference path: src/dsaX_beamformer.cu promoter (:170-193), beamformer
+// (:267-416), badder (:78-112) and adder (:120-158), fused into one launch.
+//
+// Why.  On gfx950 a 32x32 MFMA hides only a few VALU instructions issued
+// beside it (about 4 FP32 ops behind a 32-cycle FP16 MFMA, 8 behind a 64-cycle
+// FP8 one); beyond that VALU time adds to MFMA time.  In the classic schedule
+// each of the 4 waves of a tile turns the same packed voltages into its own
+// operands, and unpack + |B|^2 detect come to 6-12 VALU instructions per MFMA.
+// Here the 4 waves of a tile (one 64-beam group each, both 32-sample units,
+// as in the classic schedule) share the unpack: per (channel, pol) each thread
+// converts a quarter's worth of packed bytes into A_re operands in LDS, and
+// every wave derives A_im from A_re with XORs and moves (A_im = [Di, Dr] is a
+// swap-and-negate of A_re = [Dr, -Di]).  That brings the VALU work to ~4 (FP16)
+// and ~8 (FP8) instructions per MFMA.
+//
+// LDS stays within 80 KB, so two workgroups share a CU and cover each other's
+// staging and barriers: the packed stage uses 64-byte rows whose 8-byte chunks
+// are XOR-swizzled by row (conflict-free unpack reads), and the A_re buffer
+// is double-buffered (FP16 per 32-sample unit, FP8 per (channel, pol); 2 x 8
+// KB either way).  Results are bit-identical to the classic schedule.
+#include "bf_common.h"
+
+namespace {
+constexpr int NW = 4, NT = NW * 64;        // waves, threads per workgroup
+constexpr int BSET = NW * BTW * 32;        // beams per tile
+constexpr int SROW = 64;                   // packed stage row bytes (swizzled)
+constexpr int SSTAGE = 16 * TCU * SROW;    // 64 KB
+constexpr int AB16 = 2 * 8 * 64 * 16;      // FP16 A_re per (channel, pol): 16 KB
+constexpr int AB8 = 2 * 2 * 64 * 32;       // FP8 A_re per (channel, pol): 8 KB
+
+// Byte offset of 8-byte chunk c (0..7) of staged row r: chunks are XORed with
+// bits 2..4 of the row so that the 32 rows a unpack instruction reads fall on
+// distinct banks.
+__device__ __forceinline__ int srow(int r, int c) { return r * SROW + ((c ^ ((r >> 2) & 7)) << 3); }
+}
+
+// 4x4 transposes of 4 antennas' 32-byte chunks of packet pl into the 32 rows
+// they feed (swizzled stage; dword aq*4 of a row is in chunk aq >> 1).
+__device__ __forceinline__ void shared_stage_store(uint8_t *stage, const uint32_t (&v)[4][8], int pl, int aq) {
+  const int c = aq >> 1, w = (aq & 1) * 4;
+#pragma unroll
+  for (int ci = 0; ci < 8; ci++) {
+    const uint32_t x0 = v[0][ci], x1 = v[1][ci], x2 = v[2][ci], x3 = v[3][ci];
+    const uint32_t l01 = __builtin_amdgcn_perm(x1, x0, 0x05010400u);
+    const uint32_t h01 = __builtin_amdgcn_perm(x1, x0, 0x07030602u);
+    const uint32_t l23 = __builtin_amdgcn_perm(x3, x2, 0x05010400u);
+    const uint32_t h23 = __builtin_amdgcn_perm(x3, x2, 0x07030602u);
+    const int r0 = (ci * 2 + 0) * TCU + 2 * pl, r1 = (ci * 2 + 1) * TCU + 2 * pl;
+    *(uint32_t *)(stage + srow(r0, c) + w) = __builtin_amdgcn_perm(l23, l01, 0x05040100u);
+    *(uint32_t *)(stage + srow(r1, c) + w) = __builtin_amdgcn_perm(l23, l01, 0x07060302u);
+    *(uint32_t *)(stage + srow(r0 + 1, c) + w) = __builtin_amdgcn_perm(h23, h01, 0x05040100u);
+    *(uint32_t *)(stage + srow(r1 + 1, c) + w) = __builtin_amdgcn_perm(h23, h01, 0x07060302u);
+  }
+}
+
+// A_re operands into LDS.
+//  FP16, one unit u = 2*cq + tt at a time: step s, lane l at (s*64 + l)*16
+//        (8 halves); thread t covers lane t & 63 and steps 2*(t>>6), +1.
+//  FP8, both units of (channel, pol) cq: unit tt, step s2, lane l at
+//        ((tt*2 + s2)*64 + l)*32 (32 bytes); thread t covers lane t & 63 of
+//        unit (t>>6)&1 and the 16 antennas of step t>>7.
+// Lanes are the fastest index, so the writes are contiguous.
+__device__ __forceinline__ void unpack_unit_f16(const uint8_t *stage, uint8_t *abuf, int u, int tid) {
+  const int li = tid & 63, n = li & 31, h = li >> 5, sp = tid >> 6;
+  const int r = (u >> 1) * TCU + (u & 1) * 32 + n;
+  const uint2 x = *(const uint2 *)(stage + srow(r, h * 4 + sp));
+  h8v re0, im0, re1, im1;
+  unpack_f16(x.x, re0, im0);
+  unpack_f16(x.y, re1, im1);
+  *(h8v *)(abuf + ((2 * sp) * 64 + li) * 16) = re0;
+  *(h8v *)(abuf + ((2 * sp + 1) * 64 + li) * 16) = re1;
+}
+
+__device__ __forceinline__ void unpack_cq_f8(const uint8_t *stage, uint8_t *abuf, int cq, int tid) {
+  const int li = tid & 63, n = li & 31, h = li >> 5;
+  const int tt = (tid >> 6) & 1, q = tid >> 7;
+  const int r = cq * TCU + tt * 32 + n;
+  const uint2 x0 = *(const uint2 *)(stage + srow(r, h * 4 + 2 * q));
+  const uint2 x1 = *(const uint2 *)(stage + srow(r, h * 4 + 2 * q + 1));
+  const uint32_t x[4] = {x0.x, x0.y, x1.x, x1.y};
+  uint32_t o[8];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    o[2 * k] = nib_e4m3(x[k]);
+    o[2 * k + 1] = nib_e4m3(x[k] >> 4) ^ 0x80808080u;
+  }
+  uint8_t *p = abuf + ((tt * 2 + q) * 64 + li) * 32;
+  *(uint4 *)p = make_uint4(o[0], o[1], o[2], o[3]);
+  *(uint4 *)(p + 16) = make_uint4(o[4], o[5], o[6], o[7]);
+}
+
+// One unit from LDS A_re: the same MFMA chains, in the same order, as
+// gemm_unit.  With PF, each K step of W is reloaded from row wnext right after
+// its last MFMA has issued (a rolling prefetch of the next (channel, pol)'s
+// weights that needs no extra registers).
+template <int PREC, bool PF>
+__device__ __forceinline__ void gemm_unit_lds(Acc<BTW> &A, WFrag<PREC, BTW, 1> &W, const uint8_t *unit,
+                                              int lane, const uint8_t *wfrag, long wnext) {
+  __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+  for (int bt = 0; bt < BTW; bt++) { A.r[bt] = (f16v){}; A.i[bt] = (f16v){}; }
+  if constexpr (PREC == 0) {
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      const uint4 r = *(const uint4 *)(unit + (s * 64 + lane) * 16);
+      // A_re = [Dr0,Dr1,-Di0,-Di1,Dr2,Dr3,-Di2,-Di3] -> A_im = [Di0,Di1,Dr0,Dr1,...]
+      const uint4 m = make_uint4(r.y ^ 0x80008000u, r.x, r.w ^ 0x80008000u, r.z);
+      const h8v are = __builtin_bit_cast(h8v, r), aim = __builtin_bit_cast(h8v, m);
+#pragma unroll
+      for (int bt = 0; bt < BTW; bt++) {
+        A.r[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(are, W.w[bt][0][s], A.r[bt], 0, 0, 0);
+        A.i[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aim, W.w[bt][0][s], A.i[bt], 0, 0, 0);
+      }
+      if constexpr (PF) {
+        // pinned between this step's MFMAs and the next's (VALU, SALU and
+        // LDS may still move across)
+        __builtin_amdgcn_sched_barrier(0x386);
+        W.load_step(wfrag, wnext, s, lane);
+        __builtin_amdgcn_sched_barrier(0x386);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int s2 = 0; s2 < 2; s2++) {
+      const uint4 r0 = *(const uint4 *)(unit + (s2 * 64 + lane) * 32);
+      const uint4 r1 = *(const uint4 *)(unit + (s2 * 64 + lane) * 32 + 16);
+      // A_re dwords [Dr, -Di] per 4 antennas -> A_im [Di, Dr]
+      const i8v are = {(int)r0.x, (int)r0.y, (int)r0.z, (int)r0.w,
+                       (int)r1.x, (int)r1.y, (int)r1.z, (int)r1.w};
+      const i8v aim = {(int)(r0.y ^ 0x80808080u), (int)r0.x, (int)(r0.w ^ 0x80808080u), (int)r0.z,
+                       (int)(r1.y ^ 0x80808080u), (int)r1.x, (int)(r1.w ^ 0x80808080u), (int)r1.z};
+#pragma unroll
+      for (int bt = 0; bt < BTW; bt++) {
+        A.r[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.hi[bt][0][s2], A.r[bt], 0, 0, 0, 0, 0, 0);
+        A.i[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.hi[bt][0][s2], A.i[bt], 0, 0, 0, 0, 0, 0);
+      }
+#pragma unroll
+      for (int bt = 0; bt < BTW; bt++) {
+        A.r[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.lo[bt][0][s2], A.r[bt], 0, 0, 0, 0, 0, 0);
+        A.i[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.lo[bt][0][s2], A.i[bt], 0, 0, 0, 0, 0, 0);
+      }
+      if constexpr (PF) {
+        __builtin_amdgcn_sched_barrier(0x386);
+        W.load_step(wfrag, wnext, s2, lane);
+        __builtin_amdgcn_sched_barrier(0x386);
+      }
+    }
+  }
+  __builtin_amdgcn_s_setprio(0);
+}
+
+template <int PREC, int TAVG>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void bf_shared_kernel(BfArgs a) {
+  constexpr int UNIT = PREC == 0 ? AB16 / 2 : AB8 / 2;    // operand bytes per unit
+  constexpr int ABUF = PREC == 0 ? 2 * UNIT : 2 * AB8;     // double buffers
+  __shared__ __attribute__((aligned(16))) uint8_t stage[SSTAGE];
+  __shared__ __attribute__((aligned(16))) uint8_t abuf[ABUF];
+  constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;
+
+  // XCD-aware order as in the classic schedule (tile_id explains it)
+  const int P = blockIdx.x;
+  const int L = (P & 7) * (gridDim.x >> 3) + (P >> 3);
+  if (L >= a.ntiles) return;
+  const TileId t = tile_id(a, L);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int n = lane & 31, h = lane >> 5;
+  const int btw = (t.bset * NW + wave) * BTW;
+
+  // (a) stage: two (packet, antenna quad) tasks per thread
+  for (int task = tid; task < PCU * 16; task += NT) {
+    const int pl = task >> 4, aq = task & 15;
+    const long p = (long)t.tblk * PCU + pl;
+    uint32_t x[4][8];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int ant = aq * 4 + j;
+      if (ant < a.nant) {
+        const uint4 *src = (const uint4 *)(a.in + chunk_offset(a.order, p, ant, t.g, a.nant, a.nchan,
+                                                               a.ngroup));
+        const uint4 v0 = src[0], v1 = src[1];
+        x[j][0] = v0.x; x[j][1] = v0.y; x[j][2] = v0.z; x[j][3] = v0.w;
+        x[j][4] = v1.x; x[j][5] = v1.y; x[j][6] = v1.z; x[j][7] = v1.w;
+      } else {
+#pragma unroll
+        for (int w = 0; w < 8; w++) x[j][w] = 0u;
+      }
+    }
+    shared_stage_store(stage, x, pl, aq);
+  }
+  __syncthreads();
+
+  float part[2][BTW][NP];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int bt = 0; bt < BTW; bt++)
+#pragma unroll
+      for (int j = 0; j < NP; j++) part[i][bt][j] = 0.f;
+
+  // (b) operands to LDS one step ahead in a double buffer (FP16: per unit,
+  // 8 KB halves; FP8: per (channel, pol), 8 KB halves), so each wave's unpack
+  // of the next step interleaves with its MFMAs of this one and one barrier
+  // per step publishes the next half and frees this one
+  // Weights: (channel, pol) 0 loaded here; the second unit of each (channel,
+  // pol) prefetches the next one's in place (the last one reloads itself).
+  WFrag<PREC, BTW, 1> W;
+  {
+    const long wr = weight_row(a, t.g, 0);
+    W.load(a.wfrag, a.wscale, wr * a.nbt + btw, wr * a.nbeam_pad + btw * 32, lane, n);
+  }
+  if constexpr (PREC == 0) {
+    unpack_unit_f16(stage, abuf, 0, tid);
+    __syncthreads();
+    for (int cq = 0; cq < 16; cq++) {
+      const long wn = weight_row(a, t.g, cq < 15 ? cq + 1 : cq);
+      Acc<BTW> A;
+      unpack_unit_f16(stage, abuf + UNIT, 2 * cq + 1, tid);
+      gemm_unit_lds<PREC, false>(A, W, abuf, lane, a.wfrag, 0);
+      detect<TAVG>(A, W.sc, part[0]);
+      __syncthreads();
+      if (cq < 15) unpack_unit_f16(stage, abuf, 2 * cq + 2, tid);
+      gemm_unit_lds<PREC, true>(A, W, abuf + UNIT, lane, a.wfrag, wn * a.nbt + btw);
+      detect<TAVG>(A, W.sc, part[1]);
+      W.load_scale(a.wscale, wn * a.nbeam_pad + btw * 32, n);
+      __syncthreads();
+    }
+  } else {
+    unpack_cq_f8(stage, abuf, 0, tid);
+    __syncthreads();
+    for (int cq = 0; cq < 16; cq++) {
+      const long wn = weight_row(a, t.g, cq < 15 ? cq + 1 : cq);
+      if (cq < 15) unpack_cq_f8(stage, abuf + ((cq + 1) & 1) * AB8, cq + 1, tid);
+      const uint8_t *ab = abuf + (cq & 1) * AB8;
+      Acc<BTW> A;
+      gemm_unit_lds<PREC, false>(A, W, ab, lane, a.wfrag, 0);
+      detect<TAVG>(A, W.sc, part[0]);
+      gemm_unit_lds<PREC, true>(A, W, ab + UNIT, lane, a.wfrag, wn * a.nbt + btw);
+      detect<TAVG>(A, W.sc, part[1]);
+      W.load_scale(a.wscale, wn * a.nbeam_pad + btw * 32, n);
+      __syncthreads();
+    }
+  }
+
+  // (c)
+  tile_output<TAVG>(a, t, btw, n, h, part, 0, 2);
+}
+
+template <int PREC>
+static void launch(int tavg, dim3 grid, hipStream_t s, const BfArgs &a) {
+  switch (tavg) {
+    case 8:  hipLaunchKernelGGL((bf_shared_kernel<PREC, 8>), grid, dim3(NT), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((bf_shared_kernel<PREC, 16>), grid, dim3(NT), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((bf_shared_kernel<PREC, 32>), grid, dim3(NT), 0, s, a); break;
+    default: hipLaunchKernelGGL((bf_shared_kernel<PREC, 64>), grid, dim3(NT), 0, s, a); break;
+  }
+}
+
+extern "C" int bfk_beamform_shared(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
+                                   const float *d_wscale, const float *d_scale, uint8_t *d_out,
+                                   float *d_power, float *d_bpart, void *stream) {
+  if (d->nkc != 1 || d->nant_pad != 64 || d->nsamp % TCU != 0) {
+    bfk_set_error("bfk_beamform: shared schedule needs nant <= 64 (nant_pad=%d)", d->nant_pad);
+    return -1;
+  }
+  const BfArgs a = bf_make_args(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, BSET);
+  const dim3 grid((a.ntiles + 7) / 8 * 8);
+  if (d->prec == 0) launch<0>(d->tavg, grid, (hipStream_t)stream, a);
+  else launch<1>(d->tavg, grid, (hipStream_t)stream, a);
+  return bfk_check(hipGetLastError(), "bf_shared_kernel");
+}

@@ -50,7 +50,8 @@ static int build(dsaxbf_batch *b) {
     if (rc) return rc;
     if (b->dims[i].stuff != DSAXBF_STUFF_NONE)
       return dsaxbf_fail(DSAXBF_EINVAL, "block %d: -s/-q modes are not batched", i);
-    if (b->dims[i].variant == BFK_VAR_PIPELINED_256) b->dims[i].variant = BFK_VAR_CLASSIC_256;
+    if (b->dims[i].variant == BFK_VAR_PIPELINED_256 || b->dims[i].variant == BFK_VAR_SHARED_256)
+      b->dims[i].variant = BFK_VAR_CLASSIC_256;   /* batches run classic tiles */
   }
   if (bfk_batch_prepare(b->dims, b->bufs, b->n, &b->kern))
     return dsaxbf_fail(DSAXBF_EINVAL, "%s", bfk_last_error());

@@ -282,7 +282,8 @@ int dsaxbf_time_variants(dsaxbf_plan *p, const uint8_t *d_in, const float *d_sca
   int best = keep, rc = DSAXBF_OK;
   float best_ms = -1.f;
   for (int v = 0; v < DSAXBF_NVARIANT && rc == DSAXBF_OK; v++) {
-    if (!bfk_variant_supported(&p->dims, v) || (classic_only && v == BFK_VAR_PIPELINED_256)) {
+    if (!bfk_variant_supported(&p->dims, v) ||
+        (classic_only && (v == BFK_VAR_PIPELINED_256 || v == BFK_VAR_SHARED_256))) {
       if (ms) ms[v] = -1.f;   /* not a candidate for this shape */
       continue;
     }

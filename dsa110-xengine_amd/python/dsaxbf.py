@@ -37,8 +37,10 @@ VARIANT_PIPELINED_256 = 1
 VARIANT_CLASSIC_128S = 2
 VARIANT_CLASSIC_128 = 3
 VARIANT_CLASSIC_64 = 4
-NVARIANT = 5
-VARIANT_NAMES = {0: "classic256", 1: "pipelined256", 2: "classic128s", 3: "classic128", 4: "classic64"}
+VARIANT_SHARED_256 = 5
+NVARIANT = 6
+VARIANT_NAMES = {0: "classic256", 1: "pipelined256", 2: "classic128s", 3: "classic128", 4: "classic64",
+                 5: "shared256"}
 
 ERRORS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "EIO", -5: "ESTATE", -6: "ENODEV"}
 

@@ -176,7 +176,7 @@ int dsaxbf_execute(dsaxbf_plan *plan, const uint8_t *d_in, const float *d_scale,
  * itself when given d_beam_power.  Same stream as that execute. */
 int dsaxbf_beam_power(dsaxbf_plan *plan, float *d_beam_power, void *stream);
 
-/* Variants of the fused kernel: two schedules and three tile shapes.  All
+/* Variants of the fused kernel: three schedules and three tile shapes.  All
  * give bit-identical results; which is fastest depends on the shape (few beams
  * favour small tiles) and the GPU.
  *   CLASSIC_256    one tile (256 beams x 64 samples x 8 channels) per
@@ -187,7 +187,9 @@ int dsaxbf_beam_power(dsaxbf_plan *plan, float *d_beam_power, void *stream);
  *   CLASSIC_128S   128-beam tiles: 64 beams per wave, wave pairs split the
  *                  tile's samples (nant <= 64);
  *   CLASSIC_128    128-beam tiles: 32 beams per wave (default for nant > 64);
- *   CLASSIC_64     64-beam tiles: 32 beams per wave, samples split.
+ *   CLASSIC_64     64-beam tiles: 32 beams per wave, samples split;
+ *   SHARED_256     256-beam tiles on 8 waves that unpack each (channel, pol)
+ *                  once into LDS operands shared by all waves (nant <= 64).
  * No counterpart in the reference (its five kernels per int have one fixed
  * launch shape, dsaX_beamformer.cu:955-1003). */
 enum {
@@ -196,7 +198,8 @@ enum {
   DSAXBF_VARIANT_CLASSIC_128S = 2,
   DSAXBF_VARIANT_CLASSIC_128 = 3,
   DSAXBF_VARIANT_CLASSIC_64 = 4,
-  DSAXBF_NVARIANT = 5
+  DSAXBF_VARIANT_SHARED_256 = 5,
+  DSAXBF_NVARIANT = 6
 };
 
 /* Select the variant used by later executes on this plan; DSAXBF_EINVAL if

@@ -168,7 +168,8 @@ def test_variants_bit_identical(bf, orc, prec, nant, nchan, npkt, nbeam, order, 
     block = bf.synth_block(cfg, nbeam + tavg)
     geo = (antpos, cal, freqs, theta)
     scale = np.full(nbeam, 2.0 ** -12, np.float32)
-    variants = [v for v in range(bf.NVARIANT) if nant <= 64 or v not in (0, 1, 2)]
+    variants = [v for v in range(bf.NVARIANT)
+                if nant <= 64 or v in (bf.VARIANT_CLASSIC_128, bf.VARIANT_CLASSIC_64)]
     ref = [_run_variant(bf, cfg, block, geo, scale, v) for v in variants]
     for k in range(1, len(variants)):
         for a, b in zip(ref[0], ref[k]):
@@ -433,6 +434,7 @@ def test_batch_graph_matches_per_block(bf):
     ms = batch.autotune(2)
     assert ms.shape == (len(shapes), bf.NVARIANT)
     assert (ms[:, bf.VARIANT_PIPELINED_256] < 0).all()   # batches run classic tiles only
+    assert (ms[:, bf.VARIANT_SHARED_256] < 0).all()
     for rep in range(2):
         clear()
         batch.launch()
