@@ -209,6 +209,10 @@ template <int NB, int NK> struct WFrag<0, NB, NK> {
     for (int bt = 0; bt < NB; bt++)
       w[bt][0][s] = *(const h8v *)(wfrag + (((wrow_bt + bt) * 8 + s) * 64 + lane) * 16);
   }
+  __device__ __forceinline__ void load_one(const uint8_t *wfrag, long wrow_bt, int bt, int s, int lane) {
+    static_assert(NK == 1, "rolling prefetch is for single-chunk tiles");
+    w[bt][0][s] = *(const h8v *)(wfrag + (((wrow_bt + bt) * 8 + s) * 64 + lane) * 16);
+  }
   __device__ __forceinline__ void load_scale(const float *wscale, long sc_off, int n) {
 #pragma unroll
     for (int bt = 0; bt < NB; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
@@ -259,6 +263,12 @@ template <int NB, int NK> struct WFrag<1, NB, NK> {
       hi[bt][0][s2] = *(const i8v *)(wfrag + rec * 32);
       lo[bt][0][s2] = *(const i8v *)(wfrag + (rec + 64) * 32);
     }
+  }
+  __device__ __forceinline__ void load_one(const uint8_t *wfrag, long wrow_bt, int bt, int s2, int lane) {
+    static_assert(NK == 1, "rolling prefetch is for single-chunk tiles");
+    const long rec = (((wrow_bt + bt) * 2 + s2) * 2) * 64 + lane;
+    hi[bt][0][s2] = *(const i8v *)(wfrag + rec * 32);
+    lo[bt][0][s2] = *(const i8v *)(wfrag + (rec + 64) * 32);
   }
   __device__ __forceinline__ void load_scale(const float *wscale, long sc_off, int n) {
 #pragma unroll
@@ -362,25 +372,30 @@ __device__ __forceinline__ void gemm_unit(Acc<NB> &A, const WFrag<PREC, NB, NK> 
 // |B|^2 summed over the accumulator rows of each bin (four independent FMA
 // chains per bin), scaled into the partial power.  Lane (n, h) of a 32x32
 // accumulator holds beam column n, rows 8*(v/4) + 4h + v%4.
-template <int TAVG, int NB>
-__device__ __forceinline__ void detect(const Acc<NB> &A, const float (&sc)[NB],
-                                       float (&part)[NB][TAVG <= 32 ? 32 / TAVG : 1]) {
+template <int TAVG>
+__device__ __forceinline__ void detect1(const f16v &r, const f16v &i, float sc,
+                                        float (&part)[TAVG <= 32 ? 32 / TAVG : 1]) {
   constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;
   constexpr int VPB = 16 / NP;
 #pragma unroll
-  for (int bt = 0; bt < NB; bt++)
+  for (int j = 0; j < NP; j++) {
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
 #pragma unroll
-    for (int j = 0; j < NP; j++) {
-      float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-#pragma unroll
-      for (int v = j * VPB; v < (j + 1) * VPB; v += 2) {
-        s0 = __builtin_fmaf(A.r[bt][v], A.r[bt][v], s0);
-        s1 = __builtin_fmaf(A.i[bt][v], A.i[bt][v], s1);
-        s2 = __builtin_fmaf(A.r[bt][v + 1], A.r[bt][v + 1], s2);
-        s3 = __builtin_fmaf(A.i[bt][v + 1], A.i[bt][v + 1], s3);
-      }
-      part[bt][j] = __builtin_fmaf((s0 + s1) + (s2 + s3), sc[bt], part[bt][j]);
+    for (int v = j * VPB; v < (j + 1) * VPB; v += 2) {
+      s0 = __builtin_fmaf(r[v], r[v], s0);
+      s1 = __builtin_fmaf(i[v], i[v], s1);
+      s2 = __builtin_fmaf(r[v + 1], r[v + 1], s2);
+      s3 = __builtin_fmaf(i[v + 1], i[v + 1], s3);
     }
+    part[j] = __builtin_fmaf((s0 + s1) + (s2 + s3), sc, part[j]);
+  }
+}
+
+template <int TAVG, int NB>
+__device__ __forceinline__ void detect(const Acc<NB> &A, const float (&sc)[NB],
+                                       float (&part)[NB][TAVG <= 32 ? 32 / TAVG : 1]) {
+#pragma unroll
+  for (int bt = 0; bt < NB; bt++) detect1<TAVG>(A.r[bt], A.i[bt], sc[bt], part[bt]);
 }
 
 // 4x4 byte transposes: the NCI dwords (each 1 channel x 2 times x 2 pols) of

@@ -93,64 +93,134 @@ __device__ __forceinline__ void unpack_cq_f8(const uint8_t *stage, uint8_t *abuf
   *(uint4 *)(p + 16) = make_uint4(o[4], o[5], o[6], o[7]);
 }
 
-// One unit from LDS A_re: the same MFMA chains, in the same order, as
-// gemm_unit.  With PF, each K step of W is reloaded from row wnext right after
-// its last MFMA has issued (a rolling prefetch of the next (channel, pol)'s
-// weights that needs no extra registers).
-template <int PREC, bool PF>
-__device__ __forceinline__ void gemm_unit_lds(Acc<BTW> &A, WFrag<PREC, BTW, 1> &W, const uint8_t *unit,
-                                              int lane, const uint8_t *wfrag, long wnext) {
+// The detect of a finished chain (its accumulators and beam scale) is
+// deferred: it runs two |B|^2 terms at a time between the next chain's MFMAs,
+// where the VALU work hides behind the matrix pipe, instead of as a block of
+// VALU after the chain that the MFMAs wait for.
+struct Pending {
+  f16v r, i;
+  float sc;
+};
+
+// Element v of the pending chain into the running bin sums, in the order and
+// with the partial sums of detect1 (even v -> sums 0/1, odd v -> 2/3), so the
+// result is bit-identical.
+template <int NP>
+__device__ __forceinline__ void det_elem(const Pending &pd, int v, float (&sd)[NP][4]) {
+  constexpr int VPB = 16 / NP;
+  const int j = v / VPB, o = 2 * (v & 1);
+  sd[j][o] = __builtin_fmaf(pd.r[v], pd.r[v], sd[j][o]);
+  sd[j][o + 1] = __builtin_fmaf(pd.i[v], pd.i[v], sd[j][o + 1]);
+  // an empty use, ordered with the scheduling barriers around it, so that the
+  // compiler keeps the two FMAs at this point of the MFMA sequence
+  asm volatile("" ::"v"(sd[j][o]), "v"(sd[j][o + 1]));
+}
+
+template <int NP>
+__device__ __forceinline__ void det_final(const float (&sd)[NP][4], float sc, float (&part)[NP]) {
+#pragma unroll
+  for (int j = 0; j < NP; j++)
+    part[j] = __builtin_fmaf((sd[j][0] + sd[j][1]) + (sd[j][2] + sd[j][3]), sc, part[j]);
+}
+
+// FP16 unit from LDS A_re: the MFMA chains of gemm_unit in the same order
+// (both beam tiles per K step), so results are bit-identical to the classic
+// schedule, A_im derived from A_re.  With PF, each K step of W is reloaded
+// from row wnext right after its last MFMA has issued (a rolling prefetch of
+// the next (channel, pol)'s weights that needs no extra registers), pinned
+// there by scheduling barriers that VALU, SALU and LDS work may still cross.
+// (For FP16 this block order measured faster than the deferred-detect chains
+// that FP8 uses: those need A_im derived once per beam tile.)
+template <bool PF>
+__device__ __forceinline__ void unit16(Acc<BTW> &A, WFrag<0, BTW, 1> &W, const uint8_t *unit, int lane,
+                                       const uint8_t *wfrag, long wnext) {
   __builtin_amdgcn_s_setprio(1);
 #pragma unroll
   for (int bt = 0; bt < BTW; bt++) { A.r[bt] = (f16v){}; A.i[bt] = (f16v){}; }
-  if constexpr (PREC == 0) {
 #pragma unroll
-    for (int s = 0; s < 8; s++) {
-      const uint4 r = *(const uint4 *)(unit + (s * 64 + lane) * 16);
-      // A_re = [Dr0,Dr1,-Di0,-Di1,Dr2,Dr3,-Di2,-Di3] -> A_im = [Di0,Di1,Dr0,Dr1,...]
-      const uint4 m = make_uint4(r.y ^ 0x80008000u, r.x, r.w ^ 0x80008000u, r.z);
-      const h8v are = __builtin_bit_cast(h8v, r), aim = __builtin_bit_cast(h8v, m);
+  for (int s = 0; s < 8; s++) {
+    const uint4 r = *(const uint4 *)(unit + (s * 64 + lane) * 16);
+    // A_re = [Dr0,Dr1,-Di0,-Di1,Dr2,Dr3,-Di2,-Di3] -> A_im = [Di0,Di1,Dr0,Dr1,...]
+    const uint4 m = make_uint4(r.y ^ 0x80008000u, r.x, r.w ^ 0x80008000u, r.z);
+    const h8v are = __builtin_bit_cast(h8v, r), aim = __builtin_bit_cast(h8v, m);
 #pragma unroll
-      for (int bt = 0; bt < BTW; bt++) {
-        A.r[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(are, W.w[bt][0][s], A.r[bt], 0, 0, 0);
-        A.i[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aim, W.w[bt][0][s], A.i[bt], 0, 0, 0);
-      }
-      if constexpr (PF) {
-        // pinned between this step's MFMAs and the next's (VALU, SALU and
-        // LDS may still move across)
-        __builtin_amdgcn_sched_barrier(0x386);
-        W.load_step(wfrag, wnext, s, lane);
-        __builtin_amdgcn_sched_barrier(0x386);
-      }
+    for (int bt = 0; bt < BTW; bt++) {
+      A.r[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(are, W.w[bt][0][s], A.r[bt], 0, 0, 0);
+      A.i[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aim, W.w[bt][0][s], A.i[bt], 0, 0, 0);
     }
-  } else {
+    if constexpr (PF) {
+      __builtin_amdgcn_sched_barrier(0x386);
 #pragma unroll
-    for (int s2 = 0; s2 < 2; s2++) {
-      const uint4 r0 = *(const uint4 *)(unit + (s2 * 64 + lane) * 32);
-      const uint4 r1 = *(const uint4 *)(unit + (s2 * 64 + lane) * 32 + 16);
-      // A_re dwords [Dr, -Di] per 4 antennas -> A_im [Di, Dr]
-      const i8v are = {(int)r0.x, (int)r0.y, (int)r0.z, (int)r0.w,
-                       (int)r1.x, (int)r1.y, (int)r1.z, (int)r1.w};
-      const i8v aim = {(int)(r0.y ^ 0x80808080u), (int)r0.x, (int)(r0.w ^ 0x80808080u), (int)r0.z,
-                       (int)(r1.y ^ 0x80808080u), (int)r1.x, (int)(r1.w ^ 0x80808080u), (int)r1.z};
-#pragma unroll
-      for (int bt = 0; bt < BTW; bt++) {
-        A.r[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.hi[bt][0][s2], A.r[bt], 0, 0, 0, 0, 0, 0);
-        A.i[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.hi[bt][0][s2], A.i[bt], 0, 0, 0, 0, 0, 0);
-      }
-#pragma unroll
-      for (int bt = 0; bt < BTW; bt++) {
-        A.r[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.lo[bt][0][s2], A.r[bt], 0, 0, 0, 0, 0, 0);
-        A.i[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.lo[bt][0][s2], A.i[bt], 0, 0, 0, 0, 0, 0);
-      }
-      if constexpr (PF) {
-        __builtin_amdgcn_sched_barrier(0x386);
-        W.load_step(wfrag, wnext, s2, lane);
-        __builtin_amdgcn_sched_barrier(0x386);
-      }
+      for (int bt = 0; bt < BTW; bt++) W.load_one(wfrag, wnext, bt, s, lane);
+      __builtin_amdgcn_sched_barrier(0x386);
     }
   }
   __builtin_amdgcn_s_setprio(0);
+}
+
+// FP8 chain: beam tile BT of one 32-sample unit, the unit's A_re / A_im for
+// both K steps given, accumulated over K in the order of gemm_unit (hi then lo
+// weight part per step), so results are bit-identical to the classic
+// schedule.  Between its MFMAs it detects the pending chain into `into`, two
+// elements per MFMA.  With PF, each K step of W[BT] is reloaded from row wnext
+// right after its last MFMA (the rolling prefetch of unit16).  Scheduling
+// barriers keep this interleave as written.
+template <int BT, bool PF, int NP>
+__device__ __forceinline__ void chain8(f16v &cr, f16v &ci, WFrag<1, BTW, 1> &W, const i8v (&are)[2],
+                                       const i8v (&aim)[2], int lane, const uint8_t *wfrag, long wnext,
+                                       const Pending &pd, float (&into)[NP]) {
+  float sd[NP][4];
+#pragma unroll
+  for (int j = 0; j < NP; j++) sd[j][0] = sd[j][1] = sd[j][2] = sd[j][3] = 0.f;
+  cr = (f16v){};
+  ci = (f16v){};
+#pragma unroll
+  for (int s2 = 0; s2 < 2; s2++) {
+    const int k = 4 * s2;   // MFMA slot of this step's first MFMA
+    cr = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are[s2], W.hi[BT][0][s2], cr, 0, 0, 0, 0, 0, 0);
+    if (k > 0) { det_elem<NP>(pd, 2 * k - 2, sd); det_elem<NP>(pd, 2 * k - 1, sd); }
+    __builtin_amdgcn_sched_barrier(0);
+    ci = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim[s2], W.hi[BT][0][s2], ci, 0, 0, 0, 0, 0, 0);
+    det_elem<NP>(pd, 2 * k, sd);
+    det_elem<NP>(pd, 2 * k + 1, sd);
+    __builtin_amdgcn_sched_barrier(0);
+    cr = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are[s2], W.lo[BT][0][s2], cr, 0, 0, 0, 0, 0, 0);
+    det_elem<NP>(pd, 2 * k + 2, sd);
+    det_elem<NP>(pd, 2 * k + 3, sd);
+    __builtin_amdgcn_sched_barrier(0);
+    ci = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim[s2], W.lo[BT][0][s2], ci, 0, 0, 0, 0, 0, 0);
+    det_elem<NP>(pd, 2 * k + 4, sd);
+    det_elem<NP>(pd, 2 * k + 5, sd);
+    if constexpr (PF) W.load_one(wfrag, wnext, BT, s2, lane);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  det_elem<NP>(pd, 14, sd);
+  det_elem<NP>(pd, 15, sd);
+  det_final<NP>(sd, pd.sc, into);
+}
+
+// One FP8 unit, both beam tiles: chain 0 detects the pending chain into
+// into0, chain 1 detects chain 0 into into1, and chain 1 is left pending.
+template <bool PF, int NP>
+__device__ __forceinline__ void unit8(Pending &pd, WFrag<1, BTW, 1> &W, const uint8_t *unit, int lane,
+                                      const uint8_t *wfrag, long wnext, float (&into0)[NP], float (&into1)[NP]) {
+  f16v cr, ci;
+  __builtin_amdgcn_s_setprio(1);
+  i8v are[2], aim[2];
+#pragma unroll
+  for (int s2 = 0; s2 < 2; s2++) {
+    const uint4 r0 = *(const uint4 *)(unit + (s2 * 64 + lane) * 32);
+    const uint4 r1 = *(const uint4 *)(unit + (s2 * 64 + lane) * 32 + 16);
+    // A_re dwords [Dr, -Di] per 4 antennas -> A_im [Di, Dr]
+    are[s2] = (i8v){(int)r0.x, (int)r0.y, (int)r0.z, (int)r0.w, (int)r1.x, (int)r1.y, (int)r1.z, (int)r1.w};
+    aim[s2] = (i8v){(int)(r0.y ^ 0x80808080u), (int)r0.x, (int)(r0.w ^ 0x80808080u), (int)r0.z,
+                    (int)(r1.y ^ 0x80808080u), (int)r1.x, (int)(r1.w ^ 0x80808080u), (int)r1.z};
+  }
+  chain8<0, PF, NP>(cr, ci, W, are, aim, lane, wfrag, wnext, pd, into0);
+  pd.r = cr; pd.i = ci; pd.sc = W.sc[0];
+  chain8<1, PF, NP>(cr, ci, W, are, aim, lane, wfrag, wnext, pd, into1);
+  __builtin_amdgcn_s_setprio(0);
+  pd.r = cr; pd.i = ci; pd.sc = W.sc[1];
 }
 
 template <int PREC, int TAVG>
@@ -172,26 +242,44 @@ void bf_shared_kernel(BfArgs a) {
   const int n = lane & 31, h = lane >> 5;
   const int btw = (t.bset * NW + wave) * BTW;
 
-  // (a) stage: two (packet, antenna quad) tasks per thread
-  for (int task = tid; task < PCU * 16; task += NT) {
-    const int pl = task >> 4, aq = task & 15;
-    const long p = (long)t.tblk * PCU + pl;
-    uint32_t x[4][8];
+  // Weights of (channel, pol) 0 first, so that their latency overlaps the
+  // staging; the second unit of each (channel, pol) prefetches the next one's
+  // in place (the last one reloads itself).
+  WFrag<PREC, BTW, 1> W;
+  {
+    const long wr = weight_row(a, t.g, 0);
+    W.load(a.wfrag, a.wscale, wr * a.nbt + btw, wr * a.nbeam_pad + btw * 32, lane, n);
+  }
+
+  // (a) stage: the two (packet, antenna quad) tasks of each thread, every
+  // load issued before the first store
+  static_assert(PCU * 16 == 2 * NT, "two staging tasks per thread");
+  {
+    uint32_t x[2][4][8];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int ant = aq * 4 + j;
-      if (ant < a.nant) {
-        const uint4 *src = (const uint4 *)(a.in + chunk_offset(a.order, p, ant, t.g, a.nant, a.nchan,
-                                                               a.ngroup));
-        const uint4 v0 = src[0], v1 = src[1];
-        x[j][0] = v0.x; x[j][1] = v0.y; x[j][2] = v0.z; x[j][3] = v0.w;
-        x[j][4] = v1.x; x[j][5] = v1.y; x[j][6] = v1.z; x[j][7] = v1.w;
-      } else {
+    for (int k = 0; k < 2; k++) {
+      const int task = tid + k * NT, pl = task >> 4, aq = task & 15;
+      const long p = (long)t.tblk * PCU + pl;
 #pragma unroll
-        for (int w = 0; w < 8; w++) x[j][w] = 0u;
+      for (int j = 0; j < 4; j++) {
+        const int ant = aq * 4 + j;
+        if (ant < a.nant) {
+          const uint4 *src = (const uint4 *)(a.in + chunk_offset(a.order, p, ant, t.g, a.nant, a.nchan,
+                                                                 a.ngroup));
+          const uint4 v0 = src[0], v1 = src[1];
+          x[k][j][0] = v0.x; x[k][j][1] = v0.y; x[k][j][2] = v0.z; x[k][j][3] = v0.w;
+          x[k][j][4] = v1.x; x[k][j][5] = v1.y; x[k][j][6] = v1.z; x[k][j][7] = v1.w;
+        } else {
+#pragma unroll
+          for (int w = 0; w < 8; w++) x[k][j][w] = 0u;
+        }
       }
     }
-    shared_stage_store(stage, x, pl, aq);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const int task = tid + k * NT;
+      shared_stage_store(stage, x[k], task >> 4, task & 15);
+    }
   }
   __syncthreads();
 
@@ -207,13 +295,6 @@ void bf_shared_kernel(BfArgs a) {
   // 8 KB halves; FP8: per (channel, pol), 8 KB halves), so each wave's unpack
   // of the next step interleaves with its MFMAs of this one and one barrier
   // per step publishes the next half and frees this one
-  // Weights: (channel, pol) 0 loaded here; the second unit of each (channel,
-  // pol) prefetches the next one's in place (the last one reloads itself).
-  WFrag<PREC, BTW, 1> W;
-  {
-    const long wr = weight_row(a, t.g, 0);
-    W.load(a.wfrag, a.wscale, wr * a.nbt + btw, wr * a.nbeam_pad + btw * 32, lane, n);
-  }
   if constexpr (PREC == 0) {
     unpack_unit_f16(stage, abuf, 0, tid);
     __syncthreads();
@@ -221,30 +302,40 @@ void bf_shared_kernel(BfArgs a) {
       const long wn = weight_row(a, t.g, cq < 15 ? cq + 1 : cq);
       Acc<BTW> A;
       unpack_unit_f16(stage, abuf + UNIT, 2 * cq + 1, tid);
-      gemm_unit_lds<PREC, false>(A, W, abuf, lane, a.wfrag, 0);
+      unit16<false>(A, W, abuf, lane, a.wfrag, 0);
       detect<TAVG>(A, W.sc, part[0]);
       __syncthreads();
       if (cq < 15) unpack_unit_f16(stage, abuf, 2 * cq + 2, tid);
-      gemm_unit_lds<PREC, true>(A, W, abuf + UNIT, lane, a.wfrag, wn * a.nbt + btw);
+      unit16<true>(A, W, abuf + UNIT, lane, a.wfrag, wn * a.nbt + btw);
       detect<TAVG>(A, W.sc, part[1]);
       W.load_scale(a.wscale, wn * a.nbeam_pad + btw * 32, n);
       __syncthreads();
     }
   } else {
+    // Chains in order (cq, unit, beam tile); each one's detect is deferred to
+    // the next chain.  The first pending chain is all zero, which adds exactly
+    // nothing; the last one is detected after the loop.
+    Pending pd;
+    pd.r = (f16v){};
+    pd.i = (f16v){};
+    pd.sc = 0.f;
     unpack_cq_f8(stage, abuf, 0, tid);
     __syncthreads();
     for (int cq = 0; cq < 16; cq++) {
       const long wn = weight_row(a, t.g, cq < 15 ? cq + 1 : cq);
       if (cq < 15) unpack_cq_f8(stage, abuf + ((cq + 1) & 1) * AB8, cq + 1, tid);
       const uint8_t *ab = abuf + (cq & 1) * AB8;
-      Acc<BTW> A;
-      gemm_unit_lds<PREC, false>(A, W, ab, lane, a.wfrag, 0);
-      detect<TAVG>(A, W.sc, part[0]);
-      gemm_unit_lds<PREC, true>(A, W, ab + UNIT, lane, a.wfrag, wn * a.nbt + btw);
-      detect<TAVG>(A, W.sc, part[1]);
+      unit8<false, NP>(pd, W, ab, lane, a.wfrag, 0, part[1][1], part[0][0]);
+      unit8<true, NP>(pd, W, ab + UNIT, lane, a.wfrag, wn * a.nbt + btw, part[0][1], part[1][0]);
       W.load_scale(a.wscale, wn * a.nbeam_pad + btw * 32, n);
       __syncthreads();
     }
+    float sd[NP][4];
+#pragma unroll
+    for (int j = 0; j < NP; j++) sd[j][0] = sd[j][1] = sd[j][2] = sd[j][3] = 0.f;
+#pragma unroll
+    for (int v = 0; v < 16; v++) det_elem<NP>(pd, v, sd);
+    det_final<NP>(sd, pd.sc, part[1][1]);
   }
 
   // (c)
