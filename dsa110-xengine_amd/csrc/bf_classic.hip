@@ -65,49 +65,91 @@ __device__ __forceinline__ void classic_tile(const BfArgs &a, int L, uint8_t *st
 #pragma unroll
       for (int j = 0; j < NP; j++) part[i][bt][j] = 0.f;
 
+  // Weights of (channel, pol) 0 first, so that their latency overlaps the
+  // staging; the last unit of each (channel, pol) prefetches the next one's in
+  // place (the last one reloads itself).  Not with 256 antennas: there the
+  // weights take 128 registers, and holding them through a staging sub-step
+  // spills, so each (channel, pol) loads its own.
+  constexpr bool PFW = NKC < 4;
+  WFrag<PREC, NB, PFW ? NKC : 1> W;    // (unused with 256 antennas)
+  if constexpr (PFW) {
+    const long wr = weight_row(a, t.g, 0);
+    W.load(a.wfrag, a.wscale, wr * a.nbt + btw, wr * a.nbeam_pad + btw * 32, lane, n);
+  }
+
+  constexpr int TPT = PCU * 16 * NKC / NT;     // staging tasks per thread
+  static_assert(TPT * NT == PCU * 16 * NKC, "whole staging tasks per thread");
+  // tasks whose loads are in flight together: all of them, but half at a time
+  // with 256 antennas, where the prefetched weights hold 128 registers
+  constexpr int TB = NKC == 4 ? TPT / 2 : TPT;
+  constexpr int TUNROLL = NKC == 4 ? 1 : TPT / TB;   // batches unrolled
   for (int sub = 0; sub < NKC; sub++) {
-    // (a) stage channels sub*NCI .. sub*NCI + NCI - 1 of the group
+    // (a) stage channels sub*NCI .. sub*NCI + NCI - 1 of the group, TB tasks'
+    // loads issued before their first store
     if (sub) __syncthreads();          // the previous sub-step's rows are read
-    for (int task = tid; task < PCU * 16 * NKC; task += NT) {
-      const int pl = task / (16 * NKC), aq = task % (16 * NKC);
-      const long p = (long)t.tblk * PCU + pl;
-      uint32_t x[4][NCI];
+#pragma unroll TUNROLL
+    for (int k0 = 0; k0 < TPT; k0 += TB) {
+      uint32_t x[TB][4][NCI];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int ant = aq * 4 + j;
-        if (ant < a.nant) {
-          const uint8_t *src = a.in + chunk_offset(a.order, p, ant, t.g, a.nant, a.nchan, a.ngroup) +
-                               sub * NCI * 4;
-          if constexpr (NCI == 8) {
-            const uint4 v0 = ((const uint4 *)src)[0], v1 = ((const uint4 *)src)[1];
-            x[j][0] = v0.x; x[j][1] = v0.y; x[j][2] = v0.z; x[j][3] = v0.w;
-            x[j][4] = v1.x; x[j][5] = v1.y; x[j][6] = v1.z; x[j][7] = v1.w;
-          } else if constexpr (NCI == 4) {
-            const uint4 v0 = *(const uint4 *)src;
-            x[j][0] = v0.x; x[j][1] = v0.y; x[j][2] = v0.z; x[j][3] = v0.w;
+      for (int k = 0; k < TB; k++) {
+        const int task = tid + (k0 + k) * NT;
+        const int pl = task / (16 * NKC), aq = task % (16 * NKC);
+        const long p = (long)t.tblk * PCU + pl;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int ant = aq * 4 + j;
+          if (ant < a.nant) {
+            const uint8_t *src = a.in + chunk_offset(a.order, p, ant, t.g, a.nant, a.nchan, a.ngroup) +
+                                 sub * NCI * 4;
+            if constexpr (NCI == 8) {
+              const uint4 v0 = ((const uint4 *)src)[0], v1 = ((const uint4 *)src)[1];
+              x[k][j][0] = v0.x; x[k][j][1] = v0.y; x[k][j][2] = v0.z; x[k][j][3] = v0.w;
+              x[k][j][4] = v1.x; x[k][j][5] = v1.y; x[k][j][6] = v1.z; x[k][j][7] = v1.w;
+            } else if constexpr (NCI == 4) {
+              const uint4 v0 = *(const uint4 *)src;
+              x[k][j][0] = v0.x; x[k][j][1] = v0.y; x[k][j][2] = v0.z; x[k][j][3] = v0.w;
+            } else {
+              const uint2 v0 = *(const uint2 *)src;
+              x[k][j][0] = v0.x; x[k][j][1] = v0.y;
+            }
           } else {
-            const uint2 v0 = *(const uint2 *)src;
-            x[j][0] = v0.x; x[j][1] = v0.y;
-          }
-        } else {
 #pragma unroll
-          for (int w = 0; w < NCI; w++) x[j][w] = 0u;
+            for (int w = 0; w < NCI; w++) x[k][j][w] = 0u;
+          }
         }
       }
-      stage_store<NCI>(stage, x, pl, aq, ROWB_);
+#pragma unroll
+      for (int k = 0; k < TB; k++) {
+        const int task = tid + (k0 + k) * NT;
+        stage_store<NCI>(stage, x[k], task / (16 * NKC), task % (16 * NKC), ROWB_);
+      }
     }
     __syncthreads();
 
     // (b) CQS (channel, pol) GEMMs, one or two 32-sample units each
     for (int cql = 0; cql < CQS; cql++) {
-      const long wr = weight_row(a, t.g, sub * CQS + cql);
-      WFrag<PREC, NB, NKC> W;
-      W.load(a.wfrag, a.wscale, wr * a.nbt + btw, wr * a.nbeam_pad + btw * 32, lane, n);
+      const int cq = sub * CQS + cql;
+      if constexpr (PFW) {
+        const long wn = weight_row(a, t.g, cq < 15 ? cq + 1 : cq);
 #pragma unroll
-      for (int k = 0; k < ntt; k++) {
-        Acc<NB> A;
-        gemm_unit(A, W, stage + (cql * TCU + (tt_first + k) * 32) * ROWB_, ROWB_, n, h);
-        detect<TAVG>(A, W.sc, part[k]);
+        for (int k = 0; k < ntt; k++) {
+          Acc<NB> A;
+          const uint8_t *rows = stage + (cql * TCU + (tt_first + k) * 32) * ROWB_;
+          if (k == ntt - 1) gemm_unit<true>(A, W, rows, ROWB_, n, h, a.wfrag, wn * a.nbt + btw);
+          else gemm_unit(A, W, rows, ROWB_, n, h);
+          detect<TAVG>(A, W.sc, part[k]);
+        }
+        W.load_scale(a.wscale, wn * a.nbeam_pad + btw * 32, n);
+      } else {
+        const long wr = weight_row(a, t.g, cq);
+        WFrag<PREC, NB, NKC> Wc;
+        Wc.load(a.wfrag, a.wscale, wr * a.nbt + btw, wr * a.nbeam_pad + btw * 32, lane, n);
+#pragma unroll
+        for (int k = 0; k < ntt; k++) {
+          Acc<NB> A;
+          gemm_unit(A, Wc, stage + (cql * TCU + (tt_first + k) * 32) * ROWB_, ROWB_, n, h);
+          detect<TAVG>(A, Wc.sc, part[k]);
+        }
       }
     }
   }

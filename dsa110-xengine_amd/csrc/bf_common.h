@@ -209,9 +209,14 @@ template <int NB, int NK> struct WFrag<0, NB, NK> {
     for (int bt = 0; bt < NB; bt++)
       w[bt][0][s] = *(const h8v *)(wfrag + (((wrow_bt + bt) * 8 + s) * 64 + lane) * 16);
   }
+  // Step s of chunk kc of beam tile bt, for a rolling prefetch that refills
+  // it as soon as the last MFMA reading it has issued.
+  __device__ __forceinline__ void load_k(const uint8_t *wfrag, long wrow_bt, int bt, int kc, int s, int lane) {
+    w[bt][kc][s] = *(const h8v *)(wfrag + ((((wrow_bt + bt) * NK + kc) * 8 + s) * 64 + lane) * 16);
+  }
   __device__ __forceinline__ void load_one(const uint8_t *wfrag, long wrow_bt, int bt, int s, int lane) {
-    static_assert(NK == 1, "rolling prefetch is for single-chunk tiles");
-    w[bt][0][s] = *(const h8v *)(wfrag + (((wrow_bt + bt) * 8 + s) * 64 + lane) * 16);
+    static_assert(NK == 1, "single-chunk tiles");
+    load_k(wfrag, wrow_bt, bt, 0, s, lane);
   }
   __device__ __forceinline__ void load_scale(const float *wscale, long sc_off, int n) {
 #pragma unroll
@@ -264,11 +269,14 @@ template <int NB, int NK> struct WFrag<1, NB, NK> {
       lo[bt][0][s2] = *(const i8v *)(wfrag + (rec + 64) * 32);
     }
   }
+  __device__ __forceinline__ void load_k(const uint8_t *wfrag, long wrow_bt, int bt, int kc, int s2, int lane) {
+    const long rec = ((((wrow_bt + bt) * NK + kc) * 2 + s2) * 2) * 64 + lane;
+    hi[bt][kc][s2] = *(const i8v *)(wfrag + rec * 32);
+    lo[bt][kc][s2] = *(const i8v *)(wfrag + (rec + 64) * 32);
+  }
   __device__ __forceinline__ void load_one(const uint8_t *wfrag, long wrow_bt, int bt, int s2, int lane) {
-    static_assert(NK == 1, "rolling prefetch is for single-chunk tiles");
-    const long rec = (((wrow_bt + bt) * 2 + s2) * 2) * 64 + lane;
-    hi[bt][0][s2] = *(const i8v *)(wfrag + rec * 32);
-    lo[bt][0][s2] = *(const i8v *)(wfrag + (rec + 64) * 32);
+    static_assert(NK == 1, "single-chunk tiles");
+    load_k(wfrag, wrow_bt, bt, 0, s2, lane);
   }
   __device__ __forceinline__ void load_scale(const float *wscale, long sc_off, int n) {
 #pragma unroll
@@ -312,9 +320,14 @@ template <int NB = BTW> struct Acc {
   f16v r[NB], i[NB];
 };
 
-template <int PREC, int NB, int NK>
-__device__ __forceinline__ void gemm_unit(Acc<NB> &A, const WFrag<PREC, NB, NK> &W, const uint8_t *rows,
-                                          int rowb, int n, int h) {
+// With PF, each K step of W is reloaded from row wnext (beam tile index of
+// the next (channel, pol)) right after its last MFMA has issued: a rolling
+// prefetch that needs no extra registers, pinned there by scheduling barriers
+// that VALU, SALU and LDS work may still cross.
+template <bool PF = false, int PREC, int NB, int NK>
+__device__ __forceinline__ void gemm_unit(Acc<NB> &A, WFrag<PREC, NB, NK> &W, const uint8_t *rows,
+                                          int rowb, int n, int h, const uint8_t *wfrag = nullptr,
+                                          long wnext = 0) {
   // A wave issuing its MFMA chain runs at raised issue priority, so on a SIMD
   // shared by two waves the matrix pipe is fed first and the other wave's
   // detect/staging VALU work fills the gaps (measured: 8 % faster FP16 block,
@@ -336,6 +349,13 @@ __device__ __forceinline__ void gemm_unit(Acc<NB> &A, const WFrag<PREC, NB, NK> 
         for (int bt = 0; bt < NB; bt++) {
           A.r[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(are, W.w[bt][kc][s], A.r[bt], 0, 0, 0);
           A.i[bt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aim, W.w[bt][kc][s], A.i[bt], 0, 0, 0);
+        }
+        if constexpr (PF) {
+          const int lane = h * 32 + n;
+          __builtin_amdgcn_sched_barrier(0x386);
+#pragma unroll
+          for (int bt = 0; bt < NB; bt++) W.load_k(wfrag, wnext, bt, kc, s, lane);
+          __builtin_amdgcn_sched_barrier(0x386);
         }
       }
     } else {
@@ -362,6 +382,13 @@ __device__ __forceinline__ void gemm_unit(Acc<NB> &A, const WFrag<PREC, NB, NK> 
         for (int bt = 0; bt < NB; bt++) {
           A.r[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.lo[bt][kc][s2], A.r[bt], 0, 0, 0, 0, 0, 0);
           A.i[bt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.lo[bt][kc][s2], A.i[bt], 0, 0, 0, 0, 0, 0);
+        }
+        if constexpr (PF) {
+          const int lane = h * 32 + n;
+          __builtin_amdgcn_sched_barrier(0x386);
+#pragma unroll
+          for (int bt = 0; bt < NB; bt++) W.load_k(wfrag, wnext, bt, kc, s2, lane);
+          __builtin_amdgcn_sched_barrier(0x386);
         }
       }
     }
