@@ -173,10 +173,11 @@ def cpu_baseline(orc, cfg, block, W, scale, threads, target_s=10.0):
                 gflops=reps * FLOP_PER_BLOCK / secs / 1e9, s_per_block=secs / reps)
 
 
-def traffic_from_profiles(prec):
-    """HBM bytes per launch of the fused kernel from the committed rocprof
-    counter summary, or None when it has not been measured."""
-    path = os.path.join(ROOT, "profiles", "bf_fused_full_block_hbm_traffic.txt")
+def traffic_from_profiles(prec, kernel):
+    """HBM bytes per launch of `kernel` (bf_shared_kernel, bf_classic_kernel)
+    on the full block from its committed rocprof counter summary, or None when
+    it has not been measured."""
+    path = os.path.join(ROOT, "profiles", f"{kernel.replace('_kernel', '')}_full_block_hbm_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -426,6 +427,8 @@ def main():
     r = res[main_p]
     value = SAMPLES_PER_BLOCK / (r["ms_per_step"] / 1e3)
     achieved = FLOP_PER_BLOCK / (r["kernel_ms"] / 1e3) / 1e12
+    kernel = {"pipelined256": "bf_pipelined_kernel",
+              "shared256": "bf_shared_kernel"}.get(r["variant"], "bf_classic_kernel")
     line = {
         "metric": METRIC, "value": value, "unit": "samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
@@ -441,9 +444,8 @@ def main():
                    "realtime_fraction": r["ms_per_step"] / 134.217728},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[main_p],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[main_p],
-                     "traffic": traffic_from_profiles(main_p),
-                     "kernel": {"pipelined256": "bf_pipelined_kernel",
-                                "shared256": "bf_shared_kernel"}.get(r["variant"], "bf_classic_kernel"), "variant": r["variant"], "kernel_ms": r["kernel_ms"],
+                     "traffic": traffic_from_profiles(main_p, kernel),
+                     "kernel": kernel, "variant": r["variant"], "kernel_ms": r["kernel_ms"],
                      "flop_per_launch": FLOP_PER_BLOCK},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
                                                   ("value", "unit", "cores", "kind", "sample")},
