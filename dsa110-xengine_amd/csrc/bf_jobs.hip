@@ -4,9 +4,11 @@
 // 32-sample int, dsaX_beamformer.cu:955-1003).  A batch here holds blocks of
 // different channel, beam and time counts, each with its own weights, buffers
 // and tile shape (variant).  Blocks that share a classic kernel instantiation
-// (precision, tavg, antenna chunks, beam tiles per wave, sample split) go into one launch
-// whose workgroups walk a device table of jobs; every job keeps its own
-// BfArgs, so shapes mix freely.  bfk_batch_launch allocates nothing and never
+// (schedule, precision, tavg, antenna chunks, beam tiles per wave, sample
+// split) go into one launch whose workgroups walk a device table of jobs;
+// every job keeps its own BfArgs, so shapes mix freely.  The schedules are
+// the classic one (every tile shape) and the shared-unpack one (64 antennas,
+// 256-beam tiles).  bfk_batch_launch allocates nothing and never
 // synchronises with the host, so a caller can capture it in a HIP graph.
 #include <stdlib.h>
 #include <string.h>
@@ -21,10 +23,15 @@ extern "C" int bfk_classic_args(const bfk_dims *d, const uint8_t *d_in, const vo
 extern "C" int bfk_beamform_classic_batch(int prec, int tavg, int nkc, int nb, int split,
                                           const void *d_jobs, const int *d_tile_start, int njobs,
                                           int ntiles, void *stream);
+extern "C" int bfk_shared_args(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
+                               const float *d_wscale, const float *d_scale, uint8_t *d_out,
+                               float *d_power, float *d_bpart, BfArgs *a);
+extern "C" int bfk_beamform_shared_batch(int prec, int tavg, const void *d_jobs, const int *d_tile_start,
+                                         int njobs, int ntiles, void *stream);
 
 namespace {
 struct Group {
-  int prec, tavg, nkc, nb, split;
+  int shared, prec, tavg, nkc, nb, split;
   int njobs, ntiles;
   BfArgs *d_jobs;
   int *d_tile_start;     // njobs + 1 prefix sums
@@ -51,19 +58,24 @@ extern "C" int bfk_batch_prepare(const bfk_dims *dims, const bfk_batch_bufs *buf
     return -1;
   }
   struct Job {
-    int prec, tavg, nkc, nb;
+    int shared, prec, tavg, nkc, nb;
     BfArgs a;
   };
   std::vector<Job> jobs(n);
   for (int i = 0; i < n; i++) {
     const bfk_dims *d = &dims[i];
     if (d->variant == BFK_VAR_PIPELINED_256) {
-      bfk_set_error("bfk_batch_prepare: block %d uses the pipelined variant; batches run classic tiles", i);
+      bfk_set_error("bfk_batch_prepare: block %d uses the pipelined variant; batches run classic or "
+                    "shared tiles", i);
       return -1;
     }
     Job &j = jobs[i];
-    if (bfk_classic_args(d, bufs[i].d_in, bufs[i].d_wfrag, bufs[i].d_wscale, bufs[i].d_scale,
-                         bufs[i].d_out, bufs[i].d_power, bufs[i].d_bpart, &j.a, &j.nb))
+    j.shared = d->variant == BFK_VAR_SHARED_256;
+    j.nb = 0;
+    if (j.shared ? bfk_shared_args(d, bufs[i].d_in, bufs[i].d_wfrag, bufs[i].d_wscale, bufs[i].d_scale,
+                                   bufs[i].d_out, bufs[i].d_power, bufs[i].d_bpart, &j.a)
+                 : bfk_classic_args(d, bufs[i].d_in, bufs[i].d_wfrag, bufs[i].d_wscale, bufs[i].d_scale,
+                                    bufs[i].d_out, bufs[i].d_power, bufs[i].d_bpart, &j.a, &j.nb))
       return -1;
     j.prec = d->prec;
     j.tavg = d->tavg;
@@ -73,13 +85,14 @@ extern "C" int bfk_batch_prepare(const bfk_dims *dims, const bfk_batch_bufs *buf
   std::vector<char> done(n, 0);
   for (int i = 0; i < n; i++) {
     if (done[i]) continue;
-    Group g{jobs[i].prec, jobs[i].tavg, jobs[i].nkc, jobs[i].nb, jobs[i].a.split, 0, 0, nullptr, nullptr};
+    Group g{jobs[i].shared, jobs[i].prec, jobs[i].tavg, jobs[i].nkc, jobs[i].nb, jobs[i].a.split,
+            0, 0, nullptr, nullptr};
     std::vector<BfArgs> tab;
     std::vector<int> start{0};
     for (int k = i; k < n; k++) {
       const Job &j = jobs[k];
-      if (done[k] || j.prec != g.prec || j.tavg != g.tavg || j.nkc != g.nkc || j.nb != g.nb ||
-          j.a.split != g.split)
+      if (done[k] || j.shared != g.shared || j.prec != g.prec || j.tavg != g.tavg || j.nkc != g.nkc ||
+          j.nb != g.nb || j.a.split != g.split)
         continue;
       done[k] = 1;
       tab.push_back(j.a);
@@ -106,8 +119,10 @@ extern "C" int bfk_batch_prepare(const bfk_dims *dims, const bfk_batch_bufs *buf
 extern "C" int bfk_batch_launch(void *handle, void *stream) {
   Batch *b = (Batch *)handle;
   for (const Group &g : b->groups) {
-    const int rc = bfk_beamform_classic_batch(g.prec, g.tavg, g.nkc, g.nb, g.split, g.d_jobs,
-                                              g.d_tile_start, g.njobs, g.ntiles, stream);
+    const int rc = g.shared ? bfk_beamform_shared_batch(g.prec, g.tavg, g.d_jobs, g.d_tile_start, g.njobs,
+                                                        g.ntiles, stream)
+                            : bfk_beamform_classic_batch(g.prec, g.tavg, g.nkc, g.nb, g.split, g.d_jobs,
+                                                         g.d_tile_start, g.njobs, g.ntiles, stream);
     if (rc) return rc;
   }
   return 0;

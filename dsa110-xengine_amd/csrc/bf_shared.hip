@@ -223,19 +223,13 @@ __device__ __forceinline__ void unit8(Pending &pd, WFrag<1, BTW, 1> &W, const ui
   pd.r = cr; pd.i = ci; pd.sc = W.sc[1];
 }
 
-template <int PREC, int TAVG>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
-void bf_shared_kernel(BfArgs a) {
-  constexpr int UNIT = PREC == 0 ? AB16 / 2 : AB8 / 2;    // operand bytes per unit
-  constexpr int ABUF = PREC == 0 ? 2 * UNIT : 2 * AB8;     // double buffers
-  __shared__ __attribute__((aligned(16))) uint8_t stage[SSTAGE];
-  __shared__ __attribute__((aligned(16))) uint8_t abuf[ABUF];
-  constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;
+template <int PREC> constexpr int abuf_bytes() { return PREC == 0 ? AB16 : 2 * AB8; }   // double buffers
 
-  // XCD-aware order as in the classic schedule (tile_id explains it)
-  const int P = blockIdx.x;
-  const int L = (P & 7) * (gridDim.x >> 3) + (P >> 3);
-  if (L >= a.ntiles) return;
+// Tile L of the block described by a; stage and abuf are the workgroup's LDS.
+template <int PREC, int TAVG>
+__device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *stage, uint8_t *abuf) {
+  constexpr int UNIT = PREC == 0 ? AB16 / 2 : AB8 / 2;    // operand bytes per unit
+  constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;
   const TileId t = tile_id(a, L);
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -342,6 +336,33 @@ void bf_shared_kernel(BfArgs a) {
   tile_output<TAVG>(a, t, btw, n, h, part, 0, 2);
 }
 
+template <int PREC, int TAVG>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void bf_shared_kernel(BfArgs a) {
+  __shared__ __attribute__((aligned(16))) uint8_t stage[SSTAGE];
+  __shared__ __attribute__((aligned(16))) uint8_t abuf[abuf_bytes<PREC>()];
+  // XCD-aware order as in the classic schedule (tile_id explains it)
+  const int P = blockIdx.x;
+  const int L = (P & 7) * (gridDim.x >> 3) + (P >> 3);
+  if (L >= a.ntiles) return;
+  shared_tile<PREC, TAVG>(a, L, stage, abuf);
+}
+
+// Batched: the tiles of njobs blocks back to back; job j owns global tiles
+// [tile_start[j], tile_start[j+1]).
+template <int PREC, int TAVG>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void bf_shared_batch_kernel(const BfArgs *__restrict__ jobs, const int *__restrict__ tile_start, int njobs) {
+  __shared__ __attribute__((aligned(16))) uint8_t stage[SSTAGE];
+  __shared__ __attribute__((aligned(16))) uint8_t abuf[abuf_bytes<PREC>()];
+  const int P = blockIdx.x;
+  const int L = (P & 7) * (gridDim.x >> 3) + (P >> 3);
+  if (L >= tile_start[njobs]) return;
+  int j = 0;
+  while (L >= tile_start[j + 1]) j++;
+  shared_tile<PREC, TAVG>(jobs[j], L - tile_start[j], stage, abuf);
+}
+
 template <int PREC>
 static void launch(int tavg, dim3 grid, hipStream_t s, const BfArgs &a) {
   switch (tavg) {
@@ -352,16 +373,47 @@ static void launch(int tavg, dim3 grid, hipStream_t s, const BfArgs &a) {
   }
 }
 
+template <int PREC>
+static void launch_batch(int tavg, dim3 grid, hipStream_t s, const BfArgs *jobs, const int *ts, int nj) {
+  switch (tavg) {
+    case 8:  hipLaunchKernelGGL((bf_shared_batch_kernel<PREC, 8>), grid, dim3(NT), 0, s, jobs, ts, nj); break;
+    case 16: hipLaunchKernelGGL((bf_shared_batch_kernel<PREC, 16>), grid, dim3(NT), 0, s, jobs, ts, nj); break;
+    case 32: hipLaunchKernelGGL((bf_shared_batch_kernel<PREC, 32>), grid, dim3(NT), 0, s, jobs, ts, nj); break;
+    default: hipLaunchKernelGGL((bf_shared_batch_kernel<PREC, 64>), grid, dim3(NT), 0, s, jobs, ts, nj); break;
+  }
+}
+
+static int shared_shape_ok(const bfk_dims *d) {
+  if (d->nkc != 1 || d->nant_pad != 64 || d->nsamp % TCU != 0) {
+    bfk_set_error("bfk_beamform: shared schedule needs nant <= 64 (nant_pad=%d)", d->nant_pad);
+    return 0;
+  }
+  return 1;
+}
+
 extern "C" int bfk_beamform_shared(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
                                    const float *d_wscale, const float *d_scale, uint8_t *d_out,
                                    float *d_power, float *d_bpart, void *stream) {
-  if (d->nkc != 1 || d->nant_pad != 64 || d->nsamp % TCU != 0) {
-    bfk_set_error("bfk_beamform: shared schedule needs nant <= 64 (nant_pad=%d)", d->nant_pad);
-    return -1;
-  }
+  if (!shared_shape_ok(d)) return -1;
   const BfArgs a = bf_make_args(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, BSET);
   const dim3 grid((a.ntiles + 7) / 8 * 8);
   if (d->prec == 0) launch<0>(d->tavg, grid, (hipStream_t)stream, a);
   else launch<1>(d->tavg, grid, (hipStream_t)stream, a);
   return bfk_check(hipGetLastError(), "bf_shared_kernel");
+}
+
+// The kernel arguments of one block for a batch job table.
+extern "C" int bfk_shared_args(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag, const float *d_wscale,
+                               const float *d_scale, uint8_t *d_out, float *d_power, float *d_bpart, BfArgs *a) {
+  if (!shared_shape_ok(d)) return -1;
+  *a = bf_make_args(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, BSET);
+  return 0;
+}
+
+extern "C" int bfk_beamform_shared_batch(int prec, int tavg, const void *d_jobs, const int *d_tile_start,
+                                         int njobs, int ntiles, void *stream) {
+  const dim3 grid((ntiles + 7) / 8 * 8);
+  if (prec == 0) launch_batch<0>(tavg, grid, (hipStream_t)stream, (const BfArgs *)d_jobs, d_tile_start, njobs);
+  else launch_batch<1>(tavg, grid, (hipStream_t)stream, (const BfArgs *)d_jobs, d_tile_start, njobs);
+  return bfk_check(hipGetLastError(), "bf_shared_batch_kernel");
 }

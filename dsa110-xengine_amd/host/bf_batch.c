@@ -7,8 +7,8 @@
  * weights and device buffers) and replays, per launch, one beamforming kernel
  * per kernel instantiation in use (all tiles of all blocks sharing it) and one
  * beam-power sum per block that asks for it, all captured once in a graph.
- * The autotuner picks each block shape's tile shape by timing the classic
- * variants on that block's own buffers.
+ * The autotuner picks each block shape's variant by timing the classic and
+ * shared ones on that block's own buffers.
  */
 #include <hip/hip_runtime_api.h>
 #include <stdlib.h>
@@ -50,8 +50,8 @@ static int build(dsaxbf_batch *b) {
     if (rc) return rc;
     if (b->dims[i].stuff != DSAXBF_STUFF_NONE)
       return dsaxbf_fail(DSAXBF_EINVAL, "block %d: -s/-q modes are not batched", i);
-    if (b->dims[i].variant == BFK_VAR_PIPELINED_256 || b->dims[i].variant == BFK_VAR_SHARED_256)
-      b->dims[i].variant = BFK_VAR_CLASSIC_256;   /* batches run classic tiles */
+    if (b->dims[i].variant == BFK_VAR_PIPELINED_256)
+      b->dims[i].variant = BFK_VAR_CLASSIC_256;   /* batches run classic or shared tiles */
   }
   if (bfk_batch_prepare(b->dims, b->bufs, b->n, &b->kern))
     return dsaxbf_fail(DSAXBF_EINVAL, "%s", bfk_last_error());

@@ -265,7 +265,7 @@ int dsaxbf_plan_internals(dsaxbf_plan *p, bfk_dims *dims, void **d_wfrag, float 
 }
 
 int dsaxbf_time_variants(dsaxbf_plan *p, const uint8_t *d_in, const float *d_scale, uint8_t *d_out,
-                         int reps, int classic_only, float *ms, int *best_out) {
+                         int reps, int batchable, float *ms, int *best_out) {
   if (!p || !d_in || !d_scale || !d_out) return dsaxbf_fail(DSAXBF_EINVAL, "NULL argument");
   if (reps < 1) return dsaxbf_fail(DSAXBF_EINVAL, "reps must be >= 1");
   if (!p->have_weights && p->dims.stuff != DSAXBF_STUFF_ANTS)
@@ -283,7 +283,7 @@ int dsaxbf_time_variants(dsaxbf_plan *p, const uint8_t *d_in, const float *d_sca
   float best_ms = -1.f;
   for (int v = 0; v < DSAXBF_NVARIANT && rc == DSAXBF_OK; v++) {
     if (!bfk_variant_supported(&p->dims, v) ||
-        (classic_only && (v == BFK_VAR_PIPELINED_256 || v == BFK_VAR_SHARED_256))) {
+        (batchable && v == BFK_VAR_PIPELINED_256)) {
       if (ms) ms[v] = -1.f;   /* not a candidate for this shape */
       continue;
     }

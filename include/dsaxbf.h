@@ -288,8 +288,8 @@ int dsaxbf_batch_create(dsaxbf_batch **batch, dsaxbf_plan *const *plans, int n,
                         const uint8_t *const *d_in, const float *const *d_scale,
                         uint8_t *const *d_out, float *const *d_beam_power);
 /* Autotuner for a batch: for each distinct block shape, time every classic
- * variant on that block's buffers, set the fastest on the plans of that
- * shape, then rebuild the graph.  ms, if not NULL, receives n*DSAXBF_NVARIANT
+ * and shared variant on that block's buffers, set the fastest on the plans of
+ * that shape, then rebuild the graph.  ms, if not NULL, receives n*DSAXBF_NVARIANT
  * times (row i for block i, -1 where not run). */
 int dsaxbf_batch_autotune(dsaxbf_batch *batch, int reps, float *ms);
 /* Replay the batch graph on stream (NULL = the batch's own stream). */

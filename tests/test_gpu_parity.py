@@ -433,12 +433,28 @@ def test_batch_graph_matches_per_block(bf):
         plans[0].execute(bufs[0][0], bufs[0][1], bufs[0][2], None, bufs[0][3])   # eager work between
     ms = batch.autotune(2)
     assert ms.shape == (len(shapes), bf.NVARIANT)
-    assert (ms[:, bf.VARIANT_PIPELINED_256] < 0).all()   # batches run classic tiles only
-    assert (ms[:, bf.VARIANT_SHARED_256] < 0).all()
+    assert (ms[:, bf.VARIANT_PIPELINED_256] < 0).all()   # batches run classic and shared tiles
+    for i, sh in enumerate(shapes):     # shared: 64 antennas or fewer (repeated shapes: not re-timed)
+        if sh not in shapes[:i]:
+            assert (ms[i, bf.VARIANT_SHARED_256] > 0) == (sh[0] <= 64), i
+        else:
+            assert (ms[i] < 0).all(), i
     for rep in range(2):
         clear()
         batch.launch()
         check(f"tuned replay {rep}")
+    batch.close()
+    # every block that can take it on the shared schedule, the rest classic:
+    # one shared launch per (precision, tavg) next to the classic ones
+    for p_, sh in zip(plans, shapes):
+        p_.variant = bf.VARIANT_SHARED_256 if sh[0] <= 64 else bf.VARIANT_CLASSIC_128
+    batch = bf.Batch(plans, [b[0] for b in bufs], [b[1] for b in bufs], [b[2] for b in bufs],
+                     [b[3] for b in bufs])
+    for rep in range(2):
+        clear()
+        batch.launch()
+        check(f"shared replay {rep}")
+        plans[1].execute(bufs[1][0], bufs[1][1], bufs[1][2], None, bufs[1][3])
     batch.close()
     for p in plans:
         p.close()
