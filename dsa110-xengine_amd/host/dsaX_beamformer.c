@@ -11,8 +11,9 @@
  * build has no psrdada library, so blocks come from a file or stdin (-I) and
  * go to a file or stdout (-O); a leading DADA ASCII header (HDR_SIZE bytes,
  * 4096 by default) is copied to the output verbatim, as the reference copies
- * the ring header (:839-866).  The psrdada binding a maintainer would add is
- * shown in INTEGRATION.md.
+ * the ring header (:839-866).  A psrdada binding replaces read_full/fwrite
+ * with the reference's ipcio_open_block_read / ipcio_write sequence
+ * (:814-866, :942, :1071, :1088) around the same library calls.
  *
  * Extra options (reference defaults when absent):
  *   -I file   input blocks ("-" = stdin)      -O file  output blocks ("-" = stdout)

@@ -1,6 +1,6 @@
 """GPU parity: the HIP path through the C ABI against the FP64 oracle.
 
-Tolerances (north star): pre-quantisation power within 1e-3 relative
+Tolerances: pre-quantisation power within 1e-3 relative
 (FP16 path; the FP8-split path is held to 3e-3 per element and reported),
 uint8 within +-1 LSB everywhere (distance modulo the uint8 wrap), with the
 fraction of bytes off by one reported.  Integer-valued outputs (-s, -q) are
