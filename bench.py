@@ -201,6 +201,8 @@ def stream_bench(dsaxbf, ev, nblocks=8, seed=77):
     freqs = dsaxbf.reference_freqs(1498.75, nc)
     plan = dsaxbf.Plan(cfg)
     plan.set_weights(antpos, cal, freqs, theta)
+    tune = plan.autotune(2)         # the library's autotuner picks the kernel variant
+    variant = dsaxbf.VARIANT_NAMES[plan.variant]
     pipe = dsaxbf.Pipe(plan)
     t0 = time.time()
     for slot in range(2):
@@ -251,7 +253,9 @@ def stream_bench(dsaxbf, ev, nblocks=8, seed=77):
     wbytes = nc * 2 * (nb // 32) * (na // 64) * 64 * 128
     return {
         "workload": "256 ant x 2 pol x 384 ch x 4096 samples (805 MB packed) -> 1024 beams over "
-                    "+-3 deg, fine-channel weights, tavg 32, FP8-split, classic 128-beam tiles",
+                    "+-3 deg, fine-channel weights, tavg 32, FP8-split",
+        "variant": variant,
+        "autotune_ms": {dsaxbf.VARIANT_NAMES[i]: float(t) for i, t in enumerate(tune) if t > 0},
         "blocks_streamed": nblocks, "distinct_blocks": 2, "synth_s": synth_s,
         "s_per_block": wall, "realtime_margin": 0.134217728 / wall,
         "h2d_ms": h2d_ms, "h2d_GB_s": nin / (h2d_ms / 1e3) / 1e9,

@@ -267,12 +267,16 @@ extern "C" int bfk_beamform_classic(const bfk_dims *d, const uint8_t *d_in, cons
 extern "C" int bfk_beamform_shared(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
                                    const float *d_wscale, const float *d_scale, uint8_t *d_out,
                                    float *d_power, float *d_bpart, void *stream);
+extern "C" int bfk_beamform_shared8(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
+                                   const float *d_wscale, const float *d_scale, uint8_t *d_out,
+                                   float *d_power, float *d_bpart, void *stream);
 
 extern "C" int bfk_classic_beams(int variant, int nkc);
 
 extern "C" int bfk_variant_supported(const bfk_dims *d, int v) {
   if (d->nant_pad != 64 * d->nkc || (d->nkc != 1 && d->nkc != 2 && d->nkc != 4)) return 0;
-  if (v == BFK_VAR_PIPELINED_256 || v == BFK_VAR_SHARED_256) return d->nkc == 1;
+  if (v == BFK_VAR_PIPELINED_256) return d->nkc == 1;
+  if (v == BFK_VAR_SHARED_256) return 1;       // 4-wave tiles to 64 antennas, 8-wave above
   return bfk_classic_beams(v, d->nkc) > 0;
 }
 
@@ -316,6 +320,7 @@ extern "C" int bfk_beamform(const bfk_dims *d, const uint8_t *d_in, const void *
   if (d->variant == BFK_VAR_PIPELINED_256)
     return beamform_pipelined(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, stream);
   if (d->variant == BFK_VAR_SHARED_256)
-    return bfk_beamform_shared(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, stream);
+    return d->nkc == 1 ? bfk_beamform_shared(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, stream)
+                       : bfk_beamform_shared8(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, stream);
   return bfk_beamform_classic(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, stream);
 }

@@ -7,8 +7,8 @@
 // (schedule, precision, tavg, antenna chunks, beam tiles per wave, sample
 // split) go into one launch whose workgroups walk a device table of jobs;
 // every job keeps its own BfArgs, so shapes mix freely.  The schedules are
-// the classic one (every tile shape) and the shared-unpack one (64 antennas,
-// 256-beam tiles).  bfk_batch_launch allocates nothing and never
+// the classic one (every tile shape) and the shared-unpack one (256-beam
+// tiles: 4 waves to 64 antennas, 8 waves above).  bfk_batch_launch allocates nothing and never
 // synchronises with the host, so a caller can capture it in a HIP graph.
 #include <stdlib.h>
 #include <string.h>
@@ -28,6 +28,11 @@ extern "C" int bfk_shared_args(const bfk_dims *d, const uint8_t *d_in, const voi
                                float *d_power, float *d_bpart, BfArgs *a);
 extern "C" int bfk_beamform_shared_batch(int prec, int tavg, const void *d_jobs, const int *d_tile_start,
                                          int njobs, int ntiles, void *stream);
+extern "C" int bfk_shared8_args(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
+                                const float *d_wscale, const float *d_scale, uint8_t *d_out,
+                                float *d_power, float *d_bpart, BfArgs *a);
+extern "C" int bfk_beamform_shared8_batch(int prec, int tavg, int nkc, const void *d_jobs,
+                                          const int *d_tile_start, int njobs, int ntiles, void *stream);
 
 namespace {
 struct Group {
@@ -72,11 +77,14 @@ extern "C" int bfk_batch_prepare(const bfk_dims *dims, const bfk_batch_bufs *buf
     Job &j = jobs[i];
     j.shared = d->variant == BFK_VAR_SHARED_256;
     j.nb = 0;
-    if (j.shared ? bfk_shared_args(d, bufs[i].d_in, bufs[i].d_wfrag, bufs[i].d_wscale, bufs[i].d_scale,
-                                   bufs[i].d_out, bufs[i].d_power, bufs[i].d_bpart, &j.a)
-                 : bfk_classic_args(d, bufs[i].d_in, bufs[i].d_wfrag, bufs[i].d_wscale, bufs[i].d_scale,
-                                    bufs[i].d_out, bufs[i].d_power, bufs[i].d_bpart, &j.a, &j.nb))
-      return -1;
+    const bfk_batch_bufs &u = bufs[i];
+    const int rc = !j.shared ? bfk_classic_args(d, u.d_in, u.d_wfrag, u.d_wscale, u.d_scale, u.d_out,
+                                                u.d_power, u.d_bpart, &j.a, &j.nb)
+                   : d->nkc == 1 ? bfk_shared_args(d, u.d_in, u.d_wfrag, u.d_wscale, u.d_scale, u.d_out,
+                                                   u.d_power, u.d_bpart, &j.a)
+                                 : bfk_shared8_args(d, u.d_in, u.d_wfrag, u.d_wscale, u.d_scale, u.d_out,
+                                                    u.d_power, u.d_bpart, &j.a);
+    if (rc) return -1;
     j.prec = d->prec;
     j.tavg = d->tavg;
     j.nkc = d->nkc;
@@ -119,10 +127,12 @@ extern "C" int bfk_batch_prepare(const bfk_dims *dims, const bfk_batch_bufs *buf
 extern "C" int bfk_batch_launch(void *handle, void *stream) {
   Batch *b = (Batch *)handle;
   for (const Group &g : b->groups) {
-    const int rc = g.shared ? bfk_beamform_shared_batch(g.prec, g.tavg, g.d_jobs, g.d_tile_start, g.njobs,
-                                                        g.ntiles, stream)
-                            : bfk_beamform_classic_batch(g.prec, g.tavg, g.nkc, g.nb, g.split, g.d_jobs,
-                                                         g.d_tile_start, g.njobs, g.ntiles, stream);
+    const int rc = !g.shared ? bfk_beamform_classic_batch(g.prec, g.tavg, g.nkc, g.nb, g.split, g.d_jobs,
+                                                          g.d_tile_start, g.njobs, g.ntiles, stream)
+                   : g.nkc == 1 ? bfk_beamform_shared_batch(g.prec, g.tavg, g.d_jobs, g.d_tile_start, g.njobs,
+                                                            g.ntiles, stream)
+                                : bfk_beamform_shared8_batch(g.prec, g.tavg, g.nkc, g.d_jobs, g.d_tile_start,
+                                                             g.njobs, g.ntiles, stream);
     if (rc) return rc;
   }
   return 0;

@@ -169,7 +169,8 @@ def test_variants_bit_identical(bf, orc, prec, nant, nchan, npkt, nbeam, order, 
     geo = (antpos, cal, freqs, theta)
     scale = np.full(nbeam, 2.0 ** -12, np.float32)
     variants = [v for v in range(bf.NVARIANT)
-                if nant <= 64 or v in (bf.VARIANT_CLASSIC_128, bf.VARIANT_CLASSIC_64)]
+                if nant <= 64 or v in (bf.VARIANT_CLASSIC_128, bf.VARIANT_CLASSIC_64,
+                                       bf.VARIANT_SHARED_256)]
     ref = [_run_variant(bf, cfg, block, geo, scale, v) for v in variants]
     for k in range(1, len(variants)):
         for a, b in zip(ref[0], ref[k]):
@@ -434,9 +435,9 @@ def test_batch_graph_matches_per_block(bf):
     ms = batch.autotune(2)
     assert ms.shape == (len(shapes), bf.NVARIANT)
     assert (ms[:, bf.VARIANT_PIPELINED_256] < 0).all()   # batches run classic and shared tiles
-    for i, sh in enumerate(shapes):     # shared: 64 antennas or fewer (repeated shapes: not re-timed)
+    for i, sh in enumerate(shapes):     # repeated shapes are not re-timed
         if sh not in shapes[:i]:
-            assert (ms[i, bf.VARIANT_SHARED_256] > 0) == (sh[0] <= 64), i
+            assert ms[i, bf.VARIANT_SHARED_256] > 0, i
         else:
             assert (ms[i] < 0).all(), i
     for rep in range(2):
@@ -444,10 +445,10 @@ def test_batch_graph_matches_per_block(bf):
         batch.launch()
         check(f"tuned replay {rep}")
     batch.close()
-    # every block that can take it on the shared schedule, the rest classic:
-    # one shared launch per (precision, tavg) next to the classic ones
-    for p_, sh in zip(plans, shapes):
-        p_.variant = bf.VARIANT_SHARED_256 if sh[0] <= 64 else bf.VARIANT_CLASSIC_128
+    # every block on the shared schedule (4-wave tiles to 64 antennas, 8-wave
+    # above): one shared launch per kernel instantiation
+    for p_ in plans:
+        p_.variant = bf.VARIANT_SHARED_256
     batch = bf.Batch(plans, [b[0] for b in bufs], [b[1] for b in bufs], [b[2] for b in bufs],
                      [b[3] for b in bufs])
     for rep in range(2):
