@@ -60,10 +60,12 @@ __device__ __forceinline__ void shared_stage_store(uint8_t *stage, const uint32_
 // A_re operands into LDS.
 //  FP16, one unit u = 2*cq + tt at a time: step s, lane l at (s*64 + l)*16
 //        (8 halves); thread t covers lane t & 63 and steps 2*(t>>6), +1.
-//  FP8, both units of (channel, pol) cq: unit tt, step s2, lane l at
-//        ((tt*2 + s2)*64 + l)*32 (32 bytes); thread t covers lane t & 63 of
-//        unit (t>>6)&1 and the 16 antennas of step t>>7.
-// Lanes are the fastest index, so the writes are contiguous.
+//  FP8, both units of (channel, pol) cq: unit tt, step s2, lane l: the 32
+//        bytes in two 16-byte planes, half hl at (((tt*2 + s2)*2 + hl)*64 + l)*16;
+//        thread t covers lane t & 63 of unit (t>>6)&1 and the 16 antennas of
+//        step t>>7.
+// Lanes are the fastest index with a 16-byte stride, so the 16-byte writes
+// and reads are contiguous and free of bank conflicts.
 __device__ __forceinline__ void unpack_unit_f16(const uint8_t *stage, uint8_t *abuf, int u, int tid) {
   const int li = tid & 63, n = li & 31, h = li >> 5, sp = tid >> 6;
   const int r = (u >> 1) * TCU + (u & 1) * 32 + n;
@@ -88,9 +90,9 @@ __device__ __forceinline__ void unpack_cq_f8(const uint8_t *stage, uint8_t *abuf
     o[2 * k] = nib_e4m3(x[k]);
     o[2 * k + 1] = nib_e4m3(x[k] >> 4) ^ 0x80808080u;
   }
-  uint8_t *p = abuf + ((tt * 2 + q) * 64 + li) * 32;
+  uint8_t *p = abuf + (((tt * 2 + q) * 2) * 64 + li) * 16;
   *(uint4 *)p = make_uint4(o[0], o[1], o[2], o[3]);
-  *(uint4 *)(p + 16) = make_uint4(o[4], o[5], o[6], o[7]);
+  *(uint4 *)(p + 64 * 16) = make_uint4(o[4], o[5], o[6], o[7]);
 }
 
 // The detect of a finished chain (its accumulators and beam scale) is
@@ -209,8 +211,8 @@ __device__ __forceinline__ void unit8(Pending &pd, WFrag<1, BTW, 1> &W, const ui
   i8v are[2], aim[2];
 #pragma unroll
   for (int s2 = 0; s2 < 2; s2++) {
-    const uint4 r0 = *(const uint4 *)(unit + (s2 * 64 + lane) * 32);
-    const uint4 r1 = *(const uint4 *)(unit + (s2 * 64 + lane) * 32 + 16);
+    const uint4 r0 = *(const uint4 *)(unit + ((s2 * 2) * 64 + lane) * 16);
+    const uint4 r1 = *(const uint4 *)(unit + ((s2 * 2 + 1) * 64 + lane) * 16);
     // A_re dwords [Dr, -Di] per 4 antennas -> A_im [Di, Dr]
     are[s2] = (i8v){(int)r0.x, (int)r0.y, (int)r0.z, (int)r0.w, (int)r1.x, (int)r1.y, (int)r1.z, (int)r1.w};
     aim[s2] = (i8v){(int)(r0.y ^ 0x80808080u), (int)r0.x, (int)(r0.w ^ 0x80808080u), (int)r0.z,

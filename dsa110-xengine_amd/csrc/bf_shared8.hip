@@ -134,8 +134,9 @@ __device__ __forceinline__ void unpack8_f16(const uint8_t *stage, uint8_t *abuf,
 }
 
 // FP8 A_re of both units of sub-step (channel, pol) cql: unit tt, chunk kc,
-// step s2, lane l at (((tt*NKC + kc)*2 + s2)*64 + l)*32.  Thread t covers
-// lane t & 63 and NKC/2 of those (unit, chunk, step) combinations.
+// step s2, lane l: record C = (tt*NKC + kc)*2 + s2, its 32 bytes in two
+// 16-byte planes, half hl at ((C*2 + hl)*64 + l)*16 (16-byte lane stride:
+// conflict-free).  Thread t covers lane t & 63 and NKC/2 of the records.
 template <int NKC>
 __device__ __forceinline__ void unpack8_f8(const uint8_t *stage, uint8_t *abuf, int cql, int tid) {
   const int li = tid & 63, n = li & 31, h = li >> 5, q = tid >> 6;
@@ -151,9 +152,9 @@ __device__ __forceinline__ void unpack8_f8(const uint8_t *stage, uint8_t *abuf, 
       o[2 * k] = nib_e4m3(x[k]);
       o[2 * k + 1] = nib_e4m3(x[k] >> 4) ^ 0x80808080u;
     }
-    uint8_t *p = abuf + (C * 64 + li) * 32;
+    uint8_t *p = abuf + ((C * 2) * 64 + li) * 16;
     *(uint4 *)p = make_uint4(o[0], o[1], o[2], o[3]);
-    *(uint4 *)(p + 16) = make_uint4(o[4], o[5], o[6], o[7]);
+    *(uint4 *)(p + 64 * 16) = make_uint4(o[4], o[5], o[6], o[7]);
   }
 }
 
@@ -186,8 +187,8 @@ __device__ __forceinline__ void unit8w(Acc<1> &A, WFrag<PREC, 1, NKC> &W, const 
     } else {
 #pragma unroll
       for (int s2 = 0; s2 < 2; s2++) {
-        const uint8_t *src = unit + ((kc * 2 + s2) * 64 + lane) * 32;
-        const uint4 r0 = *(const uint4 *)src, r1 = *(const uint4 *)(src + 16);
+        const uint8_t *src = unit + (((kc * 2 + s2) * 2) * 64 + lane) * 16;
+        const uint4 r0 = *(const uint4 *)src, r1 = *(const uint4 *)(src + 64 * 16);
         // A_re dwords [Dr, -Di] per 4 antennas -> A_im [Di, Dr]
         const i8v are = {(int)r0.x, (int)r0.y, (int)r0.z, (int)r0.w, (int)r1.x, (int)r1.y, (int)r1.z, (int)r1.w};
         const i8v aim = {(int)(r0.y ^ 0x80808080u), (int)r0.x, (int)(r0.w ^ 0x80808080u), (int)r0.z,
