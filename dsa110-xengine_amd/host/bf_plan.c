@@ -58,7 +58,12 @@ static void fill_dims(const dsaxbf_config *c, bfk_dims *d) {
   d->prec = c->precision;
   d->order = c->input_order;
   d->stuff = c->stuffants;
-  d->variant = d->nkc == 1 ? BFK_VAR_CLASSIC_256 : BFK_VAR_CLASSIC_128;
+  /* the autotuner's usual picks on MI355X: shared 256-beam tiles from 129
+   * beams (full block: 64 antennas, 256 beams; 256 antennas, 1024 beams),
+   * 128- or 64-beam classic tiles below */
+  d->variant = c->nbeam > 128 ? BFK_VAR_SHARED_256
+               : c->nbeam > 64 ? (d->nkc == 1 ? BFK_VAR_CLASSIC_128S : BFK_VAR_CLASSIC_128)
+                               : BFK_VAR_CLASSIC_64;
 }
 
 void dsaxbf_destroy(dsaxbf_plan *p) {

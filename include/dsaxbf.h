@@ -180,16 +180,19 @@ int dsaxbf_beam_power(dsaxbf_plan *plan, float *d_beam_power, void *stream);
  * give bit-identical results; which is fastest depends on the shape (few beams
  * favour small tiles) and the GPU.
  *   CLASSIC_256    one tile (256 beams x 64 samples x 8 channels) per
- *                  workgroup, two workgroups per CU (default for nant <= 64);
+ *                  workgroup, two workgroups per CU (nant <= 64);
  *   PIPELINED_256  one persistent workgroup per CU with the next tile's
  *                  staging and the next weights' loads overlapped with the MFMA
  *                  work (nant <= 64);
  *   CLASSIC_128S   128-beam tiles: 64 beams per wave, wave pairs split the
- *                  tile's samples (nant <= 64);
- *   CLASSIC_128    128-beam tiles: 32 beams per wave (default for nant > 64);
- *   CLASSIC_64     64-beam tiles: 32 beams per wave, samples split;
- *   SHARED_256     256-beam tiles on 8 waves that unpack each (channel, pol)
- *                  once into LDS operands shared by all waves (nant <= 64).
+ *                  tile's samples (nant <= 64; default for 65-128 beams);
+ *   CLASSIC_128    128-beam tiles: 32 beams per wave (default for 65-128
+ *                  beams when nant > 64);
+ *   CLASSIC_64     64-beam tiles: 32 beams per wave, samples split (default
+ *                  for <= 64 beams);
+ *   SHARED_256     256-beam tiles whose waves unpack each unit once into LDS
+ *                  operands they all read: 4 waves of 64 beams (nant <= 64) or
+ *                  8 waves of 32 beams (nant > 64) (default above 128 beams).
  * No counterpart in the reference (its five kernels per int have one fixed
  * launch shape, dsaX_beamformer.cu:955-1003). */
 enum {

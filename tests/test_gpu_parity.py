@@ -198,10 +198,15 @@ def test_autotune_picks_a_variant(bf):
         plan.variant = bf.NVARIANT
     plan.close()
     big = bf.Plan(bf.config(nant=100, nchan=16, npkt=128, nbeam=256))
+    assert big.variant == bf.VARIANT_SHARED_256         # default above 128 beams
     with pytest.raises(bf.BeamformerError):
         big.variant = bf.VARIANT_PIPELINED_256          # needs nant <= 64
-    assert big.variant == bf.VARIANT_CLASSIC_128
+    assert big.variant == bf.VARIANT_SHARED_256
     big.close()
+    for nb, want in ((64, bf.VARIANT_CLASSIC_64), (128, bf.VARIANT_CLASSIC_128S)):
+        small = bf.Plan(bf.config(nant=64, nchan=16, npkt=128, nbeam=nb))
+        assert small.variant == want, nb
+        small.close()
 
 
 def test_beam_power_matches_oracle(bf, orc):
