@@ -147,6 +147,10 @@ static void *orc_worker(void *arg) {
 
 void orc_power(const orc_dims *d, int wmode, const uint8_t *in, const double *W,
                int stuff, int g0, int g1, double *power) {
+  /* groups outside the block are not computed (nothing is read or written for them) */
+  if (g0 < 0) g0 = 0;
+  if (g1 > d->nchan / 8) g1 = d->nchan / 8;
+  if (g1 <= g0) return;
   int nthr = (int)sysconf(_SC_NPROCESSORS_ONLN);
   if (nthr < 1) nthr = 1;
   if (nthr > g1 - g0) nthr = g1 - g0;

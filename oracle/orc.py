@@ -95,7 +95,8 @@ def weights(d: Dims, wmode: int, amode: int, antpos, cal, freqs, theta) -> np.nd
 def power(d: Dims, wmode: int, block, W, stuff=0, g0=0, g1=None) -> np.ndarray:
     """FP64 averaged power [nout][nbeam][ngroup]; groups outside [g0,g1) are NaN."""
     ng = d.nchan // 8
-    g1 = ng if g1 is None else g1
+    g1 = ng if g1 is None else min(g1, ng)
+    g0 = max(g0, 0)
     Wr = np.ascontiguousarray(np.stack([W.real, W.imag], -1), np.float64)
     out = np.full((2 * d.npkt // d.tavg, d.nbeam, ng), np.nan, np.float64)
     blk = np.ascontiguousarray(block, np.uint8)

@@ -157,6 +157,8 @@ def _run_variant(bf, cfg, block, geo, scale, var):
     (64, 16, 128, 130, 0, 8),
     (100, 16, 128, 64, 0, 64),     # two antenna chunks, 64 beams
     (256, 16, 128, 200, 0, 16),
+    (130, 24, 256, 300, 1, 8),     # four chunks, padded antennas, -r order, two beam sets
+    (128, 8, 128, 256, 1, 32),     # two chunks, no padding, -r order
 ])
 def test_variants_bit_identical(bf, orc, prec, nant, nchan, npkt, nbeam, order, tavg):
     """Every fused-kernel variant the shape supports gives the same bits (same
@@ -177,7 +179,7 @@ def test_variants_bit_identical(bf, orc, prec, nant, nchan, npkt, nbeam, order, 
             assert np.array_equal(a, b), bf.VARIANT_NAMES[variants[k]]
     d = orc.dims(nant, nchan, npkt, nbeam, tavg, order)
     W = orc.weights(d, 1, 1, antpos, cal, freqs, theta)
-    g = (0, 2)
+    g = (0, min(2, nchan // 8))
     want = orc.power(d, 1, block, W, 0, *g)
     r = dict(ref=want[:, :, :2], power=ref[-1][1][:, :, :2], out=ref[-1][0][:, :, :2],
              ref_u8=orc.requant_block(want, scale)[:, :, :2])

@@ -323,3 +323,18 @@ def test_agc_zero_power_and_disabled(orc, bf):
     for k in range(12):
         s, _ = p.update(base * (3 + k))       # held at the first block's median
         assert np.array_equal(s, s0)
+
+
+def test_power_group_range_is_clamped(orc, bf):
+    """A group range past the block's last group computes only the groups that
+    exist (the C restatement neither reads nor writes past the block)."""
+    nant, nchan, npkt, nbeam, tavg = 16, 8, 32, 8, 8
+    cfg = bf.config(nant=nant, nchan=nchan, npkt=npkt, nbeam=nbeam, tavg=tavg)
+    antpos, cal, theta = _geom(bf, nant, nchan, nbeam)
+    d = orc.dims(nant, nchan, npkt, nbeam, tavg)
+    W = orc.weights(d, 1, 1, antpos, cal, bf.reference_freqs(1450.0, nchan), theta)
+    block = bf.synth_block(cfg, seed=3)
+    full = orc.power(d, 1, block, W)
+    assert full.shape == (2 * npkt // tavg, nbeam, 1)
+    assert np.array_equal(orc.power(d, 1, block, W, 0, 0, 3), full)
+    assert np.array_equal(orc.power(d, 1, block, W, 0, -2, 1), full)
