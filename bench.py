@@ -169,8 +169,20 @@ def cpu_baseline(orc, cfg, block, W, scale, threads, target_s=10.0):
         reps += 1
     return dict(value=reps * SAMPLES_PER_BLOCK / secs, unit="samples/s", cores=threads, kind="port",
                 sample=f"{reps} full block(s) of the same workload, FP32 pthreads C loop "
-                       f"(oracle/bf_cpu.c), {secs:.1f} s total",
+                       f"(oracle/bf_cpu.c), {secs:.1f} s total, {threads} threads on "
+                       f"{cpu_model()} ({os.cpu_count()} logical CPUs visible)",
                 gflops=reps * FLOP_PER_BLOCK / secs / 1e9, s_per_block=secs / reps)
+
+
+def cpu_model():
+    """The host CPU's model name (lscpu's), or 'unknown CPU'."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def traffic_from_profiles(prec, kernel):
