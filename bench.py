@@ -36,6 +36,11 @@ NANT, NCHAN, NPKT, NBEAM, TAVG = 64, 384, 2048, 256, 32
 NSAMP = 2 * NPKT
 SAMPLES_PER_BLOCK = NANT * 2 * NCHAN * NSAMP
 FLOP_PER_BLOCK = 8.0 * NBEAM * NANT * NCHAN * 2 * NSAMP
+# algorithmic HBM bytes per block: voltages + fine-channel weight fragments (one
+# 128-byte record per lane, channel, pol, 32-beam tile) + uint8 output + per-32-sample
+# beam sums
+ALG_BYTES_PER_BLOCK = (SAMPLES_PER_BLOCK + NCHAN * 2 * (NBEAM // 32) * 64 * 128
+                       + (NSAMP // TAVG) * NBEAM * (NCHAN // 8) + 4 * NBEAM * (NSAMP // 32) * (NCHAN // 8))
 
 
 class HipEvents:
@@ -462,7 +467,10 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[main_p],
                      "traffic": traffic_from_profiles(main_p, kernel),
                      "kernel": kernel, "variant": r["variant"], "kernel_ms": r["kernel_ms"],
-                     "flop_per_launch": FLOP_PER_BLOCK},
+                     "flop_per_launch": FLOP_PER_BLOCK,
+                     "hbm_GB_s_algorithmic": ALG_BYTES_PER_BLOCK / (r["kernel_ms"] / 1e3) / 1e9,
+                     "hbm_GB_s_measured": None if traffic_from_profiles(main_p, kernel) is None
+                     else traffic_from_profiles(main_p, kernel) / (r["kernel_ms"] / 1e3) / 1e9},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
                                                   ("value", "unit", "cores", "kind", "sample")},
         "paths": {p: {"samples_per_s": SAMPLES_PER_BLOCK / (res[p]["ms_per_step"] / 1e3),
