@@ -238,19 +238,20 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
   const int n = lane & 31, h = lane >> 5;
   const int btw = (t.bset * NW + wave) * BTW;
 
-  // Weights of (channel, pol) 0 first, so that their latency overlaps the
-  // staging; the second unit of each (channel, pol) prefetches the next one's
-  // in place (the last one reloads itself).
+  // Weights of (channel, pol) 0 are loaded right after the staging loads:
+  // their latency overlaps the staging, and the stage stores (which wait for
+  // every older load) do not wait for them.  The second unit of each (channel,
+  // pol) prefetches the next one's in place (the last one reloads itself).
   WFrag<PREC, BTW, 1> W;
-  {
-    const long wr = weight_row(a, t.g, 0);
-    W.load(a.wfrag, a.wscale, wr * a.nbt + btw, wr * a.nbeam_pad + btw * 32, lane, n);
-  }
 
   // (a) stage: the two (packet, antenna quad) tasks of each thread, every
-  // load issued before the first store
+  // load issued before the first store.  Buffer loads without branches: a
+  // load inside a branch would be waited for at the branch's join, one round
+  // trip per antenna.
   static_assert(PCU * 16 == 2 * NT, "two staging tasks per thread");
   {
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    const __amdgpu_buffer_rsrc_t rin = buffer_rsrc(a.in, a.in_bytes);
     uint32_t x[2][4][8];
 #pragma unroll
     for (int k = 0; k < 2; k++) {
@@ -258,18 +259,22 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
       const long p = (long)t.tblk * PCU + pl;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
+        // padding antennas read past the block's end: the buffer returns zeros
         const int ant = aq * 4 + j;
-        if (ant < a.nant) {
-          const uint4 *src = (const uint4 *)(a.in + chunk_offset(a.order, p, ant, t.g, a.nant, a.nchan,
-                                                                 a.ngroup));
-          const uint4 v0 = src[0], v1 = src[1];
-          x[k][j][0] = v0.x; x[k][j][1] = v0.y; x[k][j][2] = v0.z; x[k][j][3] = v0.w;
-          x[k][j][4] = v1.x; x[k][j][5] = v1.y; x[k][j][6] = v1.z; x[k][j][7] = v1.w;
-        } else {
-#pragma unroll
-          for (int w = 0; w < 8; w++) x[k][j][w] = 0u;
-        }
+        const uint32_t off = ant < a.nant
+                                 ? (uint32_t)chunk_offset(a.order, p, ant, t.g, a.nant, a.nchan, a.ngroup)
+                                 : 0x80000000u;
+        const u4 v0 = __builtin_amdgcn_raw_buffer_load_b128(rin, off, 0, 0);
+        const u4 v1 = __builtin_amdgcn_raw_buffer_load_b128(rin, off, 16, 0);
+        x[k][j][0] = v0.x; x[k][j][1] = v0.y; x[k][j][2] = v0.z; x[k][j][3] = v0.w;
+        x[k][j][4] = v1.x; x[k][j][5] = v1.y; x[k][j][6] = v1.z; x[k][j][7] = v1.w;
       }
+    }
+    {
+      const long wr = weight_row(a, t.g, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      W.load(a.wfrag, a.wscale, wr * a.nbt + btw, wr * a.nbeam_pad + btw * 32, lane, n);
+      __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
     for (int k = 0; k < 2; k++) {
