@@ -77,12 +77,15 @@ __device__ __forceinline__ void classic_tile(const BfArgs &a, int L, uint8_t *st
     W.load(a.wfrag, a.wscale, wr * a.nbt + btw, wr * a.nbeam_pad + btw * 32, lane, n);
   }
 
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+  const __amdgpu_buffer_rsrc_t rin = buffer_rsrc(a.in, a.in_bytes);
   constexpr int TPT = PCU * 16 * NKC / NT;     // staging tasks per thread
   static_assert(TPT * NT == PCU * 16 * NKC, "whole staging tasks per thread");
-  // tasks whose loads are in flight together: all of them, but half at a time
-  // with 256 antennas, where the prefetched weights hold 128 registers
-  constexpr int TB = NKC == 4 ? TPT / 2 : TPT;
-  constexpr int TUNROLL = NKC == 4 ? 1 : TPT / TB;   // batches unrolled
+  // tasks whose loads are in flight together: all of them with 64 antennas,
+  // half at a time above, where the weights hold 64 or 128 registers
+  constexpr int TB = NKC >= 2 ? TPT / 2 : TPT;
+  constexpr int TUNROLL = NKC >= 2 ? 1 : TPT / TB;   // batches unrolled
   for (int sub = 0; sub < NKC; sub++) {
     // (a) stage channels sub*NCI .. sub*NCI + NCI - 1 of the group, TB tasks'
     // loads issued before their first store
@@ -97,24 +100,24 @@ __device__ __forceinline__ void classic_tile(const BfArgs &a, int L, uint8_t *st
         const long p = (long)t.tblk * PCU + pl;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
+          // branch-free buffer loads: padding antennas read past the block's
+          // end and get zeros (a load inside a branch would be waited for at
+          // the join, one round trip per antenna)
           const int ant = aq * 4 + j;
-          if (ant < a.nant) {
-            const uint8_t *src = a.in + chunk_offset(a.order, p, ant, t.g, a.nant, a.nchan, a.ngroup) +
-                                 sub * NCI * 4;
-            if constexpr (NCI == 8) {
-              const uint4 v0 = ((const uint4 *)src)[0], v1 = ((const uint4 *)src)[1];
-              x[k][j][0] = v0.x; x[k][j][1] = v0.y; x[k][j][2] = v0.z; x[k][j][3] = v0.w;
-              x[k][j][4] = v1.x; x[k][j][5] = v1.y; x[k][j][6] = v1.z; x[k][j][7] = v1.w;
-            } else if constexpr (NCI == 4) {
-              const uint4 v0 = *(const uint4 *)src;
-              x[k][j][0] = v0.x; x[k][j][1] = v0.y; x[k][j][2] = v0.z; x[k][j][3] = v0.w;
-            } else {
-              const uint2 v0 = *(const uint2 *)src;
-              x[k][j][0] = v0.x; x[k][j][1] = v0.y;
-            }
+          const uint32_t off =
+              ant < a.nant ? (uint32_t)chunk_offset(a.order, p, ant, t.g, a.nant, a.nchan, a.ngroup) + sub * NCI * 4
+                           : 0x80000000u;
+          if constexpr (NCI == 8) {
+            const u4 v0 = __builtin_amdgcn_raw_buffer_load_b128(rin, off, 0, 0);
+            const u4 v1 = __builtin_amdgcn_raw_buffer_load_b128(rin, off, 16, 0);
+            x[k][j][0] = v0.x; x[k][j][1] = v0.y; x[k][j][2] = v0.z; x[k][j][3] = v0.w;
+            x[k][j][4] = v1.x; x[k][j][5] = v1.y; x[k][j][6] = v1.z; x[k][j][7] = v1.w;
+          } else if constexpr (NCI == 4) {
+            const u4 v0 = __builtin_amdgcn_raw_buffer_load_b128(rin, off, 0, 0);
+            x[k][j][0] = v0.x; x[k][j][1] = v0.y; x[k][j][2] = v0.z; x[k][j][3] = v0.w;
           } else {
-#pragma unroll
-            for (int w = 0; w < NCI; w++) x[k][j][w] = 0u;
+            const u2 v0 = __builtin_amdgcn_raw_buffer_load_b64(rin, off, 0, 0);
+            x[k][j][0] = v0.x; x[k][j][1] = v0.y;
           }
         }
       }

@@ -69,6 +69,11 @@ extern "C" int bfk_batch_prepare(const bfk_dims *dims, const bfk_batch_bufs *buf
   std::vector<Job> jobs(n);
   for (int i = 0; i < n; i++) {
     const bfk_dims *d = &dims[i];
+    // buffer offsets are 32-bit, as for single blocks
+    if ((size_t)d->npkt * d->nant * d->nchan * 4 > 0x7fffffffu || bfk_wfrag_bytes(d) > 0x7fffffffu) {
+      bfk_set_error("bfk_batch_prepare: block %d or its weights above 2 GiB", i);
+      return -1;
+    }
     if (d->variant == BFK_VAR_PIPELINED_256) {
       bfk_set_error("bfk_batch_prepare: block %d uses the pipelined variant; batches run classic or "
                     "shared tiles", i);
