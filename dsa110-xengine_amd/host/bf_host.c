@@ -184,13 +184,15 @@ static int cmp_float(const void *x, const void *y) {
 
 /* ret_med_bp (:507-514): every beam gets 0.5*(bp[n/2-1] + bp[n/2]) of the
  * sorted powers. */
-static void median_fill(float *bp, int n) {
+static int median_fill(float *bp, int n) {
   float *tmp = malloc(sizeof(float) * n);
+  if (!tmp) return dsaxbf_fail(DSAXBF_ENOMEM, "AGC median: allocation failed");
   memcpy(tmp, bp, sizeof(float) * n);
   qsort(tmp, n, sizeof(float), cmp_float);
   float med = 0.5f * (tmp[n / 2 - 1] + tmp[n / 2]);
   for (int i = 0; i < n; i++) bp[i] = med;
   free(tmp);
+  return DSAXBF_OK;
 }
 
 int dsaxbf_agc_update(dsaxbf_agc *agc, const float *bp_raw, float *scale_out, int *zero_output) {
@@ -201,12 +203,13 @@ int dsaxbf_agc_update(dsaxbf_agc *agc, const float *bp_raw, float *scale_out, in
   memcpy(bp, bp_raw, sizeof(float) * nb);
   memcpy(agc->many_bp + (size_t)(agc->bpctr % DSAXBF_NBP) * nb, bp, sizeof(float) * nb);
   if (agc->started == 0) {           /* :1021-1033 first block: median, zeros out */
+    if (median_fill(bp, nb)) return DSAXBF_ENOMEM;
     agc->started = 1;
-    median_fill(bp, nb);
     agc->medbp = bp[0];
     zero = 1;
   }
-  if (agc->started > 0 && agc->bpctr < DSAXBF_NBP) median_fill(bp, nb);   /* :1035-1036 */
+  if (agc->started > 0 && agc->bpctr < DSAXBF_NBP && median_fill(bp, nb))   /* :1035-1036 */
+    return DSAXBF_ENOMEM;
   if (agc->started > 0 && agc->bpctr >= DSAXBF_NBP) {                      /* :1039-1048 */
     for (int i = 0; i < nb; i++) {   /* ret_many_bp :516-530 */
       float s = 0.f;
@@ -380,9 +383,12 @@ int dsaxbf_synth_block(const dsaxbf_config *cfg, uint64_t seed, double sigma, in
     job[i].blk = h_block;
     job[i].p0 = (int)((long)cfg->npkt * i / nthr);
     job[i].p1 = (int)((long)cfg->npkt * (i + 1) / nthr);
-    pthread_create(&tid[i], NULL, synth_worker, &job[i]);
   }
-  for (int i = 0; i < nthr; i++) pthread_join(tid[i], NULL);
+  int started = 0;   /* workers that run in threads; the rest run here */
+  for (; started < nthr; started++)
+    if (pthread_create(&tid[started], NULL, synth_worker, &job[started])) break;
+  for (int i = started; i < nthr; i++) synth_worker(&job[i]);
+  for (int i = 0; i < started; i++) pthread_join(tid[i], NULL);
   return DSAXBF_OK;
 }
 

@@ -294,12 +294,13 @@ int dsaxbf_time_variants(dsaxbf_plan *p, const uint8_t *d_in, const float *d_sca
     }
     p->dims.variant = v;
     float t = 0.f;
-    for (int r = -1; r < reps && rc == DSAXBF_OK; r++) {
+    he = hipSuccess;
+    for (int r = -1; r < reps && rc == DSAXBF_OK && he == hipSuccess; r++) {
       if (r == 0) he = hipEventRecord(e0, p->stream);
-      rc = dsaxbf_execute(p, d_in, d_scale, d_out, NULL, NULL, p->stream);
+      if (he == hipSuccess) rc = dsaxbf_execute(p, d_in, d_scale, d_out, NULL, NULL, p->stream);
     }
     if (rc) break;
-    he = hipEventRecord(e1, p->stream);
+    if (he == hipSuccess) he = hipEventRecord(e1, p->stream);
     if (he == hipSuccess) he = hipEventSynchronize(e1);
     if (he == hipSuccess) he = hipEventElapsedTime(&t, e0, e1);
     if (he != hipSuccess) {

@@ -144,7 +144,9 @@ int dsaxbf_read_weights_file(const char *fnam, const char *flagfile,
  *   theta_rad: [nbeam] beam angles (radians); NULL selects the reference
  *              fan theta_b = sep_arcmin*(nbeam/2-1-b)*pi/10800
  *              (dsaX_beamformer.cu:440, sep = 1 arcmin there).
- * Host pointers.  Synchronous. */
+ * Host pointers.  Synchronous.  The new weights are written on the plan's
+ * stream, after the executes queued there; executes queued on other streams
+ * read the same weight buffer, so the caller lets them finish first. */
 int dsaxbf_set_weights(dsaxbf_plan *plan, const float *antpos, const float *cal,
                        const double *freqs_hz, const double *theta_rad,
                        double sep_arcmin);
