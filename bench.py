@@ -10,11 +10,13 @@ channel; detect, sum 2 pols x 32 samples x 8 channels, requantise to uint8
 distinct blocks in rotation, larger than the 256 MB last-level cache) when
 the timed region starts.
 
-Both precisions run; "value" is the FP8-split path (FP8 E4M3 voltages x hi+lo
-FP8 weights, FP32 accumulate) unless --precision fp16.  The line also carries
-the FP16 path, the error of each against the FP64 oracle on a subset of
-channel groups, the MFMA roofline of the fused kernel (HIP events on the
-kernel's own stream) and a multithreaded FP32 CPU baseline.
+Three precisions run; "value" is the FP8-split path (FP8 E4M3 voltages x
+hi+lo FP8 weights, FP32 accumulate) unless --precision names another.  The
+line also carries the FP16 path and the FP6-split path (E2M3 voltages x hi+lo
+E2M3 weights, twice the FP8 matrix rate), the error of each against the FP64
+oracle on a subset of channel groups, the MFMA roofline of the fused kernel
+(HIP events on the kernel's own stream) and a multithreaded FP32 CPU
+baseline.
 """
 from __future__ import annotations
 
@@ -31,7 +33,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "dsa110-xengine_amd", "python"), os.path.join(ROOT, "oracle")]
 
 METRIC = "beamformed input samples/s + % MFMA peak, FP8-split vs FP16, at +-1 LSB"
-PEAK_TFLOPS = {"fp16": 2500.0, "fp8split": 5000.0}   # MI355X dense MFMA
+PEAK_TFLOPS = {"fp16": 2500.0, "fp8split": 5000.0, "fp6split": 10100.0}   # MI355X dense MFMA
+DTYPES = {"fp16": "fp16 x fp16, fp32 accumulate",
+          "fp8split": "fp8-e4m3 voltages x hi+lo fp8-e4m3 weights, fp32 accumulate",
+          "fp6split": "fp6-e2m3 voltages x hi+lo fp6-e2m3 weights, fp32 accumulate"}
 NANT, NCHAN, NPKT, NBEAM, TAVG = 64, 384, 2048, 256, 32
 NSAMP = 2 * NPKT
 SAMPLES_PER_BLOCK = NANT * 2 * NCHAN * NSAMP
@@ -90,7 +95,8 @@ def workload(dsaxbf, nblocks, seed=2024):
 
 def time_path(dsaxbf, ev, prec_name, cfg, antpos, cal, theta, freqs, blocks, steps, warmup,
               variant="auto"):
-    cfg.precision = dsaxbf.PREC_FP16 if prec_name == "fp16" else dsaxbf.PREC_FP8SPLIT
+    cfg.precision = {"fp16": dsaxbf.PREC_FP16, "fp8split": dsaxbf.PREC_FP8SPLIT,
+                     "fp6split": dsaxbf.PREC_FP6SPLIT}[prec_name]
     plan = dsaxbf.Plan(cfg)
     plan.set_weights(antpos, cal, freqs, theta)
     tune_ms = None
@@ -288,7 +294,7 @@ def batch_bench(dsaxbf, ev, seed=2025, nblocks=24, reps=5):
     channel counts from {48, 96, 192, 384}, beam counts from {64, 128, 256, 512}
     and time lengths from {1024, 2048, 4096} samples (seeded draws), 4-bit
     Gaussian voltages, geometry-derived fine-channel weights, tavg 32; run on
-    both precisions.  Reported: the graph replay time of the batch with the
+    each precision.  Reported: the graph replay time of the batch with the
     autotuned tile shapes, against the same blocks run one execute at a time
     with the default shape."""
     rng = np.random.default_rng(seed)
@@ -297,7 +303,8 @@ def batch_bench(dsaxbf, ev, seed=2025, nblocks=24, reps=5):
     res = {"blocks": [{"nchan": c, "nbeam": b, "nsamp": t} for c, b, t in shapes]}
     flop = sum(8.0 * b * 64 * c * 2 * t for c, b, t in shapes)
     samples = sum(64 * 2 * c * t for c, b, t in shapes)
-    for prec_name, prec in (("fp16", dsaxbf.PREC_FP16), ("fp8split", dsaxbf.PREC_FP8SPLIT)):
+    for prec_name, prec in (("fp16", dsaxbf.PREC_FP16), ("fp8split", dsaxbf.PREC_FP8SPLIT),
+                            ("fp6split", dsaxbf.PREC_FP6SPLIT)):
         plans, bufs = [], []
         for i, (c, b, t) in enumerate(shapes):
             cfg = dsaxbf.config(nant=64, nchan=c, npkt=t // 2, nbeam=b, tavg=32, precision=prec,
@@ -363,7 +370,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--precision", choices=["fp8split", "fp16"], default="fp8split")
+    ap.add_argument("--precision", choices=["fp8split", "fp16", "fp6split"], default="fp8split")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the oracle error report")
     ap.add_argument("--check-groups", type=int, default=8)
@@ -400,7 +407,7 @@ def main():
     cfg, antpos, cal, theta, freqs, blocks, tone_beam = workload(dsaxbf, 3)
     print(f"# workload ready in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     res = {}
-    for prec in ("fp16", "fp8split"):
+    for prec in ("fp16", "fp8split", "fp6split"):
         res[prec] = time_path(dsaxbf, ev, prec, cfg, antpos, cal, theta, freqs, blocks,
                               args.steps, args.warmup, args.variant)
         print(f"# {prec}: {res[prec]['ms_per_step']:.4f} ms/step, kernel "
@@ -454,8 +461,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp8-e4m3 voltages x hi+lo fp8-e4m3 weights, fp32 accumulate"
-                 if main_p == "fp8split" else "fp16 x fp16, fp32 accumulate",
+        "dtype": DTYPES[main_p],
         "data": "synthetic: seeded 4-bit complex Gaussian sigma 2 clipped +-7, tone SNR 10 in 4 "
                 "channels; geometry-derived FP64 weights (random-init equivalent)",
         "config": {"workload": "full DSA block: 64 ant x 2 pol x 384 fine ch x 4096 samples "
@@ -477,11 +483,12 @@ def main():
                       "ms_per_step": res[p]["ms_per_step"], "kernel_ms": res[p]["kernel_ms"],
                       "mfma_frac_of_own_peak": FLOP_PER_BLOCK / (res[p]["kernel_ms"] / 1e3) / 1e12
                       / PEAK_TFLOPS[p],
-                      "executed_mfma_frac": (2.0 if p == "fp8split" else 1.0) * FLOP_PER_BLOCK
+                      "executed_mfma_frac": (1.0 if p == "fp16" else 2.0) * FLOP_PER_BLOCK
                       / (res[p]["kernel_ms"] / 1e3) / 1e12 / PEAK_TFLOPS[p],
                       "variant": res[p]["variant"], "autotune_ms": res[p]["autotune_ms"],
                       "errors_vs_fp64_oracle": errs.get(p)} for p in res},
         "fp8split_speedup_vs_fp16": res["fp16"]["ms_per_step"] / res["fp8split"]["ms_per_step"],
+        "fp6split_speedup_vs_fp16": res["fp16"]["ms_per_step"] / res["fp6split"]["ms_per_step"],
         "tone_check": {k: errs[k] for k in ("tone_beam", "peak_beam_fp16") if k in errs},
         "scaled_array_stream": stream,
         "uneven_batch_graph": batch,

@@ -30,8 +30,13 @@ int bfk_check(hipError_t e, const char *what);
 //   step s2 = (a%32)/16, dword 2u = Wr of ants 4u..4u+3 of the 16,
 //   dword 2u+1 = Wi; hi and lo parts are separate 32-byte records,
 //   index ((((cq*nbt + bt)*nkc + kc)*2 + s2)*2 + hl)*64 + lane.
-// Both formats use 128 bytes per lane per (cq, beam tile, antenna chunk).
-// The matching A (voltage) slots are built by unpack_f16 / nib_e4m3.
+// FP6 (the same MFMA with E2M3 operands, 6-bit slot j at bits 6j..6j+5 of six
+//   dwords): step s2 as for FP8, slots 0..15 = Wr of the 16 antennas, slots
+//   16..31 = Wi; hi and lo records at the FP8 indices, 24 of their 32 bytes
+//   used.
+// All formats use 128 bytes per lane per (cq, beam tile, antenna chunk).
+// The matching A (voltage) slots are built by unpack_f16 / nib_e4m3 /
+// nib_e2m3 + pack_e2m3.
 // ---------------------------------------------------------------------------
 constexpr int WFRAG_LANE_BYTES = 128;
 
@@ -89,6 +94,55 @@ __device__ __forceinline__ uint32_t nib_e4m3(uint32_t xs) {
   const uint32_t sm = (m << 1) - t;        // 0x0F per negative byte
   return ((s ^ sm) + t) | (m << 4);
 }
+
+// OCP FP6 E2M3 (gfx950 "fp6"): bias 1, max 7.5, subnormal step 1/8.  Round
+// to nearest even, saturating.  Code c in 0..8 encodes c/8, so a 4-bit
+// voltage v is the sign-magnitude code (v<0)<<5 | |v| worth v/8, exactly.
+__device__ static inline uint32_t e2m3_encode(double v) {
+  uint32_t sgn = v < 0 ? 0x20u : 0u;
+  double a = fabs(v);
+  if (a >= 7.5) return sgn | 0x1Fu;
+  if (a < 1.0) return sgn | (uint32_t)rint(a * 8.0);   // 0..8 (8 = 1.0, the smallest normal)
+  int e;
+  double m = frexp(a, &e);                // a = m * 2^e, m in [0.5, 1), e in 1..3
+  double mq = rint((m * 2.0 - 1.0) * 8.0);
+  int ex = e - 1;
+  if (mq >= 8.0) { mq = 0.0; ex += 1; }
+  if (ex > 2) return sgn | 0x1Fu;
+  return sgn | ((uint32_t)(ex + 1) << 3) | (uint32_t)mq;
+}
+__device__ static inline double e2m3_decode(uint32_t c) {
+  double s = (c & 0x20u) ? -1.0 : 1.0;
+  uint32_t e = (c >> 3) & 3u, m = c & 7u;
+  if (e == 0) return s * (double)m * 0.125;
+  return s * (1.0 + m / 8.0) * ldexp(1.0, (int)e - 1);
+}
+
+// The low nibbles of 4 bytes -> 4 E2M3 codes, one per byte (as nib_e4m3,
+// with the sign at bit 5).
+__device__ __forceinline__ uint32_t nib_e2m3(uint32_t xs) {
+  const uint32_t s = xs & 0x0F0F0F0Fu;
+  const uint32_t m = xs & 0x08080808u;
+  const uint32_t t = m >> 3;
+  const uint32_t sm = (m << 1) - t;
+  return ((s ^ sm) + t) | (m << 2);
+}
+
+// 16 six-bit codes, 4 per dword in the low 6 bits of each byte (c[k] byte i
+// = slot 4k + i), packed densely into 3 dwords (slot j at bits 6j..6j+5).
+__device__ __forceinline__ void pack_e2m3(const uint32_t (&c)[4], uint32_t &d0, uint32_t &d1, uint32_t &d2) {
+  uint32_t g[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t p = ((c[k] >> 2) & 0x0FC00FC0u) | (c[k] & 0x003F003Fu);   // 12-bit pairs per half
+    g[k] = (p & 0xFFFu) | ((p >> 4) & 0xFFF000u);                            // 24 bits
+  }
+  d0 = g[0] | (g[1] << 24);
+  d1 = (g[1] >> 8) | (g[2] << 16);
+  d2 = (g[2] >> 16) | (g[3] << 8);
+}
+// Sign bits of 16 packed E2M3 slots (bits 6j+5) in each of the 3 dwords.
+constexpr uint32_t E2M3_SIGN0 = 0x20820820u, E2M3_SIGN1 = 0x08208208u, E2M3_SIGN2 = 0x82082082u;
 
 // (unsigned char)(__float2int_rn(x*scale)/2), dsaX_beamformer.cu:150-156:
 // round half to even, saturate to int, C division, wrap modulo 256.
@@ -310,6 +364,44 @@ template <int NB, int NK> struct WFrag<1, NB, NK> {
             else lo[bt][kc][s2] = v;
           }
         }
+  }
+};
+
+// FP6 hi + lo fragments: 6 dwords each (the top two of the i8v are unused).
+template <int NB, int NK> struct WFrag<2, NB, NK> {
+  i8v hi[NB][NK][2], lo[NB][NK][2];
+  float sc[NB];
+  static __device__ __forceinline__ i8v ld6(const uint8_t *p) {
+    typedef int i4v __attribute__((ext_vector_type(4)));
+    typedef int i2v __attribute__((ext_vector_type(2)));
+    typedef int i6v __attribute__((ext_vector_type(6)));
+    const i4v a = *(const i4v *)p;
+    const i2v b = *(const i2v *)(p + 16);
+    const i6v v = __builtin_shufflevector(a, __builtin_shufflevector(b, b, 0, 1, -1, -1), 0, 1, 2, 3, 4, 5);
+    return __builtin_shufflevector(v, v, 0, 1, 2, 3, 4, 5, -1, -1);
+  }
+  __device__ __forceinline__ void load_k(const uint8_t *wfrag, long wrow_bt, int bt, int kc, int s2, int lane) {
+    const long rec = ((((wrow_bt + bt) * NK + kc) * 2 + s2) * 2) * 64 + lane;
+    hi[bt][kc][s2] = ld6(wfrag + rec * 32);
+    lo[bt][kc][s2] = ld6(wfrag + (rec + 64) * 32);
+  }
+  __device__ __forceinline__ void load_scale(const float *wscale, long sc_off, int n) {
+#pragma unroll
+    for (int bt = 0; bt < NB; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
+  }
+  __device__ __forceinline__ void load(const uint8_t *wfrag, const float *wscale, long wrow_bt,
+                                       long sc_off, int lane, int n) {
+#pragma unroll
+    for (int bt = 0; bt < NB; bt++)
+#pragma unroll
+      for (int kc = 0; kc < NK; kc++)
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) load_k(wfrag, wrow_bt, bt, kc, s2, lane);
+    load_scale(wscale, sc_off, n);
+  }
+  __device__ __forceinline__ void load_one(const uint8_t *wfrag, long wrow_bt, int bt, int s2, int lane) {
+    static_assert(NK == 1, "single-chunk tiles");
+    load_k(wfrag, wrow_bt, bt, 0, s2, lane);
   }
 };
 

@@ -79,6 +79,10 @@ extern "C" int bfk_batch_prepare(const bfk_dims *dims, const bfk_batch_bufs *buf
                     "shared tiles", i);
       return -1;
     }
+    if (!bfk_variant_supported(d, d->variant)) {
+      bfk_set_error("bfk_batch_prepare: block %d: variant %d cannot run its shape", i, d->variant);
+      return -1;
+    }
     Job &j = jobs[i];
     j.shared = d->variant == BFK_VAR_SHARED_256;
     j.nb = 0;

@@ -28,7 +28,7 @@ typedef struct bfk_dims {
   int ncal;       /* weight channels: ngroup (group mode) or nchan (fine) */
   int wmode;      /* 0 group, 1 fine                                      */
   int amode;      /* 0 theta, 1 sin(theta)                                */
-  int prec;       /* 0 FP16, 1 FP8 hi+lo                                  */
+  int prec;       /* 0 FP16, 1 FP8 hi+lo, 2 FP6 hi+lo                     */
   int order;      /* 0 capture, 1 split -r                                */
   int stuff;      /* 0 none, 1 antennas, 2 incoherent beam 0              */
   int variant;    /* fused-kernel variant, BFK_VAR_*                      */

@@ -17,6 +17,11 @@
 // swap-and-negate of A_re = [Dr, -Di]).  That brings the VALU work to ~4 (FP16)
 // and ~8 (FP8) instructions per MFMA.
 //
+// FP6-split (E2M3 voltage codes v/8 and E2M3 hi + lo weights, the lo part
+// 32x larger and scaled back by the MFMA's E8M0 operand scale) runs the FP8
+// chains at twice the matrix rate: a 32x32x64 MFMA on 6-bit operands takes 32
+// cycles instead of 64.
+//
 // LDS stays within 80 KB, so two workgroups share a CU and cover each other's
 // staging and barriers: the packed stage uses 64-byte rows whose 8-byte chunks
 // are XOR-swizzled by row (conflict-free unpack reads), and the A_re buffer
@@ -95,6 +100,33 @@ __device__ __forceinline__ void unpack_cq_f8(const uint8_t *stage, uint8_t *abuf
   *(uint4 *)(p + 64 * 16) = make_uint4(o[4], o[5], o[6], o[7]);
 }
 
+// FP6, both units of (channel, pol) cq, the same thread mapping as FP8: the
+// 16 antennas of a lane's step are slots 0..15 (Dr) and 16..31 (-Di), six
+// dwords per lane in two planes, unit tt and step s2 at (tt*4 + s2*1.5) KB:
+// dwords 0..3 at lane*16, dwords 4..5 at 1024 + lane*8 (both conflict-free).
+__device__ __forceinline__ void unpack_cq_f6(const uint8_t *stage, uint8_t *abuf, int cq, int tid) {
+  const int li = tid & 63, n = li & 31, h = li >> 5;
+  const int tt = (tid >> 6) & 1, q = tid >> 7;
+  const int r = cq * TCU + tt * 32 + n;
+  const uint2 x0 = *(const uint2 *)(stage + srow(r, h * 4 + 2 * q));
+  const uint2 x1 = *(const uint2 *)(stage + srow(r, h * 4 + 2 * q + 1));
+  const uint32_t x[4] = {x0.x, x0.y, x1.x, x1.y};
+  uint32_t cr[4], ci[4], o[6];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    cr[k] = nib_e2m3(x[k]);
+    ci[k] = nib_e2m3(x[k] >> 4);
+  }
+  pack_e2m3(cr, o[0], o[1], o[2]);
+  pack_e2m3(ci, o[3], o[4], o[5]);
+  o[3] ^= E2M3_SIGN0;   // -Di
+  o[4] ^= E2M3_SIGN1;
+  o[5] ^= E2M3_SIGN2;
+  uint8_t *p = abuf + tt * 4096 + q * 1536;
+  *(uint4 *)(p + li * 16) = make_uint4(o[0], o[1], o[2], o[3]);
+  *(uint2 *)(p + 1024 + li * 8) = make_uint2(o[4], o[5]);
+}
+
 // The detect of a finished chain (its accumulators and beam scale) is
 // deferred: it runs two |B|^2 terms at a time between the next chain's MFMAs,
 // where the VALU work hides behind the matrix pipe, instead of as a block of
@@ -167,8 +199,17 @@ __device__ __forceinline__ void unit16(Acc<BTW> &A, WFrag<0, BTW, 1> &W, const u
 // elements per MFMA.  With PF, each K step of W[BT] is reloaded from row wnext
 // right after its last MFMA (the rolling prefetch of unit16).  Scheduling
 // barriers keep this interleave as written.
-template <int BT, bool PF, int NP>
-__device__ __forceinline__ void chain8(f16v &cr, f16v &ci, WFrag<1, BTW, 1> &W, const i8v (&are)[2],
+// One 32x32x64 MFMA of the FP8 (PREC 1) or FP6 (PREC 2) chains; LO marks the
+// weights' lo part, which FP6 stores 2^5 larger (E8M0 scales: A 2^0, B 2^-5).
+template <int PREC, bool LO>
+__device__ __forceinline__ f16v mfma64(const i8v &a, const i8v &b, const f16v &c) {
+  if constexpr (PREC == 1) return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, 0, 0, 0);
+  else if constexpr (!LO) return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 2, 2, 0, 0, 0, 0);
+  else return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 2, 2, 0, 127, 0, 122);
+}
+
+template <int PREC, int BT, bool PF, int NP>
+__device__ __forceinline__ void chain8(f16v &cr, f16v &ci, WFrag<PREC, BTW, 1> &W, const i8v (&are)[2],
                                        const i8v (&aim)[2], int lane, const uint8_t *wfrag, long wnext,
                                        const Pending &pd, float (&into)[NP]) {
   float sd[NP][4];
@@ -179,18 +220,18 @@ __device__ __forceinline__ void chain8(f16v &cr, f16v &ci, WFrag<1, BTW, 1> &W, 
 #pragma unroll
   for (int s2 = 0; s2 < 2; s2++) {
     const int k = 4 * s2;   // MFMA slot of this step's first MFMA
-    cr = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are[s2], W.hi[BT][0][s2], cr, 0, 0, 0, 0, 0, 0);
+    cr = mfma64<PREC, false>(are[s2], W.hi[BT][0][s2], cr);
     if (k > 0) { det_elem<NP>(pd, 2 * k - 2, sd); det_elem<NP>(pd, 2 * k - 1, sd); }
     __builtin_amdgcn_sched_barrier(0);
-    ci = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim[s2], W.hi[BT][0][s2], ci, 0, 0, 0, 0, 0, 0);
+    ci = mfma64<PREC, false>(aim[s2], W.hi[BT][0][s2], ci);
     det_elem<NP>(pd, 2 * k, sd);
     det_elem<NP>(pd, 2 * k + 1, sd);
     __builtin_amdgcn_sched_barrier(0);
-    cr = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are[s2], W.lo[BT][0][s2], cr, 0, 0, 0, 0, 0, 0);
+    cr = mfma64<PREC, true>(are[s2], W.lo[BT][0][s2], cr);
     det_elem<NP>(pd, 2 * k + 2, sd);
     det_elem<NP>(pd, 2 * k + 3, sd);
     __builtin_amdgcn_sched_barrier(0);
-    ci = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim[s2], W.lo[BT][0][s2], ci, 0, 0, 0, 0, 0, 0);
+    ci = mfma64<PREC, true>(aim[s2], W.lo[BT][0][s2], ci);
     det_elem<NP>(pd, 2 * k + 4, sd);
     det_elem<NP>(pd, 2 * k + 5, sd);
     if constexpr (PF) W.load_one(wfrag, wnext, BT, s2, lane);
@@ -201,31 +242,41 @@ __device__ __forceinline__ void chain8(f16v &cr, f16v &ci, WFrag<1, BTW, 1> &W, 
   det_final<NP>(sd, pd.sc, into);
 }
 
-// One FP8 unit, both beam tiles: chain 0 detects the pending chain into
-// into0, chain 1 detects chain 0 into into1, and chain 1 is left pending.
-template <bool PF, int NP>
-__device__ __forceinline__ void unit8(Pending &pd, WFrag<1, BTW, 1> &W, const uint8_t *unit, int lane,
+// One FP8 or FP6 unit, both beam tiles: chain 0 detects the pending chain
+// into into0, chain 1 detects chain 0 into into1, and chain 1 is left pending.
+template <int PREC, bool PF, int NP>
+__device__ __forceinline__ void unit8(Pending &pd, WFrag<PREC, BTW, 1> &W, const uint8_t *unit, int lane,
                                       const uint8_t *wfrag, long wnext, float (&into0)[NP], float (&into1)[NP]) {
   f16v cr, ci;
   __builtin_amdgcn_s_setprio(1);
   i8v are[2], aim[2];
 #pragma unroll
   for (int s2 = 0; s2 < 2; s2++) {
-    const uint4 r0 = *(const uint4 *)(unit + ((s2 * 2) * 64 + lane) * 16);
-    const uint4 r1 = *(const uint4 *)(unit + ((s2 * 2 + 1) * 64 + lane) * 16);
-    // A_re dwords [Dr, -Di] per 4 antennas -> A_im [Di, Dr]
-    are[s2] = (i8v){(int)r0.x, (int)r0.y, (int)r0.z, (int)r0.w, (int)r1.x, (int)r1.y, (int)r1.z, (int)r1.w};
-    aim[s2] = (i8v){(int)(r0.y ^ 0x80808080u), (int)r0.x, (int)(r0.w ^ 0x80808080u), (int)r0.z,
-                    (int)(r1.y ^ 0x80808080u), (int)r1.x, (int)(r1.w ^ 0x80808080u), (int)r1.z};
+    if constexpr (PREC == 1) {
+      const uint4 r0 = *(const uint4 *)(unit + ((s2 * 2) * 64 + lane) * 16);
+      const uint4 r1 = *(const uint4 *)(unit + ((s2 * 2 + 1) * 64 + lane) * 16);
+      // A_re dwords [Dr, -Di] per 4 antennas -> A_im [Di, Dr]
+      are[s2] = (i8v){(int)r0.x, (int)r0.y, (int)r0.z, (int)r0.w, (int)r1.x, (int)r1.y, (int)r1.z, (int)r1.w};
+      aim[s2] = (i8v){(int)(r0.y ^ 0x80808080u), (int)r0.x, (int)(r0.w ^ 0x80808080u), (int)r0.z,
+                      (int)(r1.y ^ 0x80808080u), (int)r1.x, (int)(r1.w ^ 0x80808080u), (int)r1.z};
+    } else {
+      const uint4 r0 = *(const uint4 *)(unit + s2 * 1536 + lane * 16);
+      const uint2 r1 = *(const uint2 *)(unit + s2 * 1536 + 1024 + lane * 8);
+      // A_re = [Dr | -Di] (16 slots each) -> A_im = [Di | Dr]; dwords 6, 7 unused
+      are[s2] = (i8v){(int)r0.x, (int)r0.y, (int)r0.z, (int)r0.w, (int)r1.x, (int)r1.y, 0, 0};
+      aim[s2] = (i8v){(int)(r0.w ^ E2M3_SIGN0), (int)(r1.x ^ E2M3_SIGN1), (int)(r1.y ^ E2M3_SIGN2),
+                      (int)r0.x, (int)r0.y, (int)r0.z, 0, 0};
+    }
   }
-  chain8<0, PF, NP>(cr, ci, W, are, aim, lane, wfrag, wnext, pd, into0);
+  chain8<PREC, 0, PF, NP>(cr, ci, W, are, aim, lane, wfrag, wnext, pd, into0);
   pd.r = cr; pd.i = ci; pd.sc = W.sc[0];
-  chain8<1, PF, NP>(cr, ci, W, are, aim, lane, wfrag, wnext, pd, into1);
+  chain8<PREC, 1, PF, NP>(cr, ci, W, are, aim, lane, wfrag, wnext, pd, into1);
   __builtin_amdgcn_s_setprio(0);
   pd.r = cr; pd.i = ci; pd.sc = W.sc[1];
 }
 
 template <int PREC> constexpr int abuf_bytes() { return PREC == 0 ? AB16 : 2 * AB8; }   // double buffers
+static_assert(2 * 4096 <= AB8, "FP6 operands of one (channel, pol) fit the FP8 half-buffer");
 
 // Tile L of the block described by a; stage and abuf are the workgroup's LDS.
 template <int PREC, int TAVG>
@@ -320,14 +371,18 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
     pd.r = (f16v){};
     pd.i = (f16v){};
     pd.sc = 0.f;
-    unpack_cq_f8(stage, abuf, 0, tid);
+    auto unpack_cq = [&](uint8_t *dst, int c) {
+      if constexpr (PREC == 1) unpack_cq_f8(stage, dst, c, tid);
+      else unpack_cq_f6(stage, dst, c, tid);
+    };
+    unpack_cq(abuf, 0);
     __syncthreads();
     for (int cq = 0; cq < 16; cq++) {
       const long wn = weight_row(a, t.g, cq < 15 ? cq + 1 : cq);
-      if (cq < 15) unpack_cq_f8(stage, abuf + ((cq + 1) & 1) * AB8, cq + 1, tid);
+      if (cq < 15) unpack_cq(abuf + ((cq + 1) & 1) * AB8, cq + 1);
       const uint8_t *ab = abuf + (cq & 1) * AB8;
-      unit8<false, NP>(pd, W, ab, lane, a.wfrag, 0, part[1][1], part[0][0]);
-      unit8<true, NP>(pd, W, ab + UNIT, lane, a.wfrag, wn * a.nbt + btw, part[0][1], part[1][0]);
+      unit8<PREC, false, NP>(pd, W, ab, lane, a.wfrag, 0, part[1][1], part[0][0]);
+      unit8<PREC, true, NP>(pd, W, ab + UNIT, lane, a.wfrag, wn * a.nbt + btw, part[0][1], part[1][0]);
       W.load_scale(a.wscale, wn * a.nbeam_pad + btw * 32, n);
       __syncthreads();
     }
@@ -405,7 +460,8 @@ extern "C" int bfk_beamform_shared(const bfk_dims *d, const uint8_t *d_in, const
   const BfArgs a = bf_make_args(d, d_in, d_wfrag, d_wscale, d_scale, d_out, d_power, d_bpart, BSET);
   const dim3 grid((a.ntiles + 7) / 8 * 8);
   if (d->prec == 0) launch<0>(d->tavg, grid, (hipStream_t)stream, a);
-  else launch<1>(d->tavg, grid, (hipStream_t)stream, a);
+  else if (d->prec == 1) launch<1>(d->tavg, grid, (hipStream_t)stream, a);
+  else launch<2>(d->tavg, grid, (hipStream_t)stream, a);
   return bfk_check(hipGetLastError(), "bf_shared_kernel");
 }
 
@@ -421,6 +477,7 @@ extern "C" int bfk_beamform_shared_batch(int prec, int tavg, const void *d_jobs,
                                          int njobs, int ntiles, void *stream) {
   const dim3 grid((ntiles + 7) / 8 * 8);
   if (prec == 0) launch_batch<0>(tavg, grid, (hipStream_t)stream, (const BfArgs *)d_jobs, d_tile_start, njobs);
-  else launch_batch<1>(tavg, grid, (hipStream_t)stream, (const BfArgs *)d_jobs, d_tile_start, njobs);
+  else if (prec == 1) launch_batch<1>(tavg, grid, (hipStream_t)stream, (const BfArgs *)d_jobs, d_tile_start, njobs);
+  else launch_batch<2>(tavg, grid, (hipStream_t)stream, (const BfArgs *)d_jobs, d_tile_start, njobs);
   return bfk_check(hipGetLastError(), "bf_shared_batch_kernel");
 }

@@ -1,7 +1,7 @@
 // bf_weights.hip -- beam weights on the device (replaces calc_weights,
 // dsaX_beamformer.cu:420-449): FP64 phase exp(-2 pi i f ang x / c) times the
 // calibration, scaled by a power of two per (channel, pol, beam) and written
-// as FP16 or FP8 E4M3 hi+lo MFMA B-operand fragments.
+// as FP16, FP8 E4M3 hi+lo or FP6 E2M3 hi+lo MFMA B-operand fragments.
 #include <stdarg.h>
 
 #include "bf_common.h"
@@ -28,8 +28,11 @@ extern "C" size_t bfk_wscale_bytes(const bfk_dims *d) {
 }
 
 // One thread per (weight channel wc, pol, beam).  The scale puts the largest
-// component of the beam's weights in [2^14, 2^15) (FP16) or [128, 256) (FP8);
-// wscale holds the power factor that undoes it (and the FP8 voltage 2^-9).
+// component of the beam's weights in [2^14, 2^15) (FP16), [128, 256) (FP8) or
+// [2, 4) (FP6); wscale holds the power factor that undoes it and the voltage
+// code's unit (2^-9 for FP8, 1/8 for FP6).  The FP6 lo part is stored 32x
+// larger (its residual is at most 1/8 there) and the MFMA's E8M0 operand
+// scale of 2^-5 takes that back.
 __global__ void bf_weights_kernel(bfk_dims d, const float *__restrict__ antpos,
                                   const float *__restrict__ cal,
                                   const double *__restrict__ freqs,
@@ -60,9 +63,11 @@ __global__ void bf_weights_kernel(bfk_dims d, const float *__restrict__ antpos,
     }
   int k = 0;
   if (mx > 0.0) frexp(mx, &k);
-  int e = (d.prec == 0 ? 15 : 8) - k;
+  int e = (d.prec == 0 ? 15 : d.prec == 1 ? 8 : 2) - k;
   double sc = ldexp(1.0, e);
-  wscale[idx] = (float)(d.prec == 0 ? ldexp(1.0, -2 * e) : ldexp(1.0, 18 - 2 * e));
+  wscale[idx] = (float)(d.prec == 0   ? ldexp(1.0, -2 * e)
+                        : d.prec == 1 ? ldexp(1.0, 18 - 2 * e)
+                                      : ldexp(1.0, 6 - 2 * e));
 
   int bt = b >> 5, n = b & 31;
   for (int kc = 0; kc < d.nkc; kc++)
@@ -90,6 +95,26 @@ __global__ void bf_weights_kernel(bfk_dims d, const float *__restrict__ antpos,
           v[6] = (_Float16)wi[j + 2]; v[7] = (_Float16)wi[j + 3];
           long rec = (((long)(cq * d.nbt + bt) * d.nkc + kc) * 8 + s) * 64 + lane;
           *(h8v *)(wfrag + rec * 16) = v;
+        }
+      } else if (d.prec == 2) {
+        for (int s2 = 0; s2 < 2; s2++) {
+          uint32_t hi[8] = {0, 0, 0, 0, 0, 0, 0, 0}, lo[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          for (int j = 0; j < 32; j++) {    // slot j: Wr (j < 16) or Wi of antenna j % 16
+            const double w = j < 16 ? wr[16 * s2 + j] : wi[16 * s2 + j - 16];
+            const uint32_t ch = e2m3_encode(w);
+            const uint32_t cl = e2m3_encode((w - e2m3_decode(ch)) * 32.0);
+            const int bit = 6 * j, dw = bit >> 5, sh = bit & 31;
+            hi[dw] |= ch << sh;
+            lo[dw] |= cl << sh;
+            if (sh > 26) {
+              hi[dw + 1] |= ch >> (32 - sh);
+              lo[dw + 1] |= cl >> (32 - sh);
+            }
+          }
+          long rec = ((((long)(cq * d.nbt + bt) * d.nkc + kc) * 2 + s2) * 2) * 64 + lane;
+          uint32_t *ph = (uint32_t *)(wfrag + rec * 32);
+          uint32_t *pl = (uint32_t *)(wfrag + (rec + 64) * 32);
+          for (int u = 0; u < 8; u++) { ph[u] = hi[u]; pl[u] = lo[u]; }
         }
       } else {
         for (int s2 = 0; s2 < 2; s2++) {

@@ -70,8 +70,11 @@ int dsaxbf_check_config(const dsaxbf_config *c) {
     return dsaxbf_fail(DSAXBF_EINVAL, "nbeam=%d outside 1..1024", c->nbeam);
   if (c->tavg != 8 && c->tavg != 16 && c->tavg != 32 && c->tavg != 64)
     return dsaxbf_fail(DSAXBF_EINVAL, "tavg=%d must be 8, 16, 32 or 64", c->tavg);
-  if (c->precision != DSAXBF_PREC_FP16 && c->precision != DSAXBF_PREC_FP8SPLIT)
+  if (c->precision != DSAXBF_PREC_FP16 && c->precision != DSAXBF_PREC_FP8SPLIT &&
+      c->precision != DSAXBF_PREC_FP6SPLIT)
     return dsaxbf_fail(DSAXBF_EINVAL, "precision=%d unknown", c->precision);
+  if (c->precision == DSAXBF_PREC_FP6SPLIT && c->nant > 64)
+    return dsaxbf_fail(DSAXBF_EINVAL, "precision FP6-split runs up to 64 antennas (nant=%d)", c->nant);
   if (c->weight_mode != 0 && c->weight_mode != 1)
     return dsaxbf_fail(DSAXBF_EINVAL, "weight_mode=%d unknown", c->weight_mode);
   if (c->angle_mode != 0 && c->angle_mode != 1)

@@ -60,8 +60,8 @@ static void fill_dims(const dsaxbf_config *c, bfk_dims *d) {
   d->stuff = c->stuffants;
   /* the autotuner's usual picks on MI355X: shared 256-beam tiles from 129
    * beams (full block: 64 antennas, 256 beams; 256 antennas, 1024 beams),
-   * 128- or 64-beam classic tiles below */
-  d->variant = c->nbeam > 128 ? BFK_VAR_SHARED_256
+   * 128- or 64-beam classic tiles below; FP6 runs shared tiles only */
+  d->variant = c->nbeam > 128 || c->precision == DSAXBF_PREC_FP6SPLIT ? BFK_VAR_SHARED_256
                : c->nbeam > 64 ? (d->nkc == 1 ? BFK_VAR_CLASSIC_128S : BFK_VAR_CLASSIC_128)
                                : BFK_VAR_CLASSIC_64;
 }

@@ -18,7 +18,7 @@
  * Extra options (reference defaults when absent):
  *   -I file   input blocks ("-" = stdin)      -O file  output blocks ("-" = stdout)
  *   -n nant   antennas per block (63)          -T tavg  samples per output (8)
- *   -P prec   fp16 | fp8split (fp16)           -W mode  group | fine (group)
+ *   -P prec   fp16 | fp8split | fp6split (fp16)   -W mode  group | fine (group)
  *   -A mode   theta | sin (theta)              -r       input is dsaX_split -r order
  *   -N n      stop after n blocks              -w       re-read weights every block
  */
@@ -48,7 +48,7 @@ static void usage(void) {
           " -t test pattern \n"
           " -h print usage\n"
           " -I infile (- = stdin)   -O outfile (- = stdout)   -n nant [63]\n"
-          " -T tavg [8]  -P fp16|fp8split  -W group|fine  -A theta|sin  -r  -N nblocks  -w\n");
+          " -T tavg [8]  -P fp16|fp8split|fp6split  -W group|fine  -A theta|sin  -r  -N nblocks  -w\n");
 }
 
 /* Read exactly n bytes unless EOF; returns bytes read. */
@@ -102,7 +102,11 @@ int main(int argc, char *argv[]) {
       case 'O': outfile = optarg; break;
       case 'n': nant = atoi(optarg); break;
       case 'T': tavg = atoi(optarg); break;
-      case 'P': prec = strcmp(optarg, "fp8split") == 0 ? DSAXBF_PREC_FP8SPLIT : DSAXBF_PREC_FP16; break;
+      case 'P':
+        prec = strcmp(optarg, "fp8split") == 0   ? DSAXBF_PREC_FP8SPLIT
+               : strcmp(optarg, "fp6split") == 0 ? DSAXBF_PREC_FP6SPLIT
+                                                 : DSAXBF_PREC_FP16;
+        break;
       case 'W': wmode = strcmp(optarg, "fine") == 0 ? DSAXBF_WEIGHTS_FINE : DSAXBF_WEIGHTS_GROUP; break;
       case 'A': amode = strcmp(optarg, "sin") == 0 ? DSAXBF_ANGLE_SIN : DSAXBF_ANGLE_THETA; break;
       case 'r': order = DSAXBF_ORDER_SPLIT_R; break;

@@ -23,6 +23,7 @@ LIB_PATH = os.environ.get("DSAXBF_LIB") or os.path.join(os.path.dirname(_HERE), 
 
 PREC_FP16 = 0
 PREC_FP8SPLIT = 1
+PREC_FP6SPLIT = 2
 WEIGHTS_GROUP = 0
 WEIGHTS_FINE = 1
 ANGLE_THETA = 0

@@ -1,7 +1,9 @@
 """GPU parity: the HIP path through the C ABI against the FP64 oracle.
 
 Tolerances: pre-quantisation power within 1e-3 relative
-(FP16 path; the FP8-split path is held to 3e-3 per element and reported),
+(FP16 path; the FP8-split path is held to 3e-3 per element and the FP6-split
+path to 5e-3 -- E2M3 has two exponent bits, so its hi+lo pair keeps less
+relative precision on small weight components -- and both are reported),
 uint8 within +-1 LSB everywhere (distance modulo the uint8 wrap), with the
 fraction of bytes off by one reported.  Integer-valued outputs (-s, -q) are
 bit-exact.
@@ -17,7 +19,7 @@ from conftest import GOLDEN, ROOT
 
 pytestmark = pytest.mark.gpu
 
-TOL = {0: 1e-3, 1: 3e-3}
+TOL = {0: 1e-3, 1: 3e-3, 2: 5e-3}
 
 
 def _u8_dist(a, b):
@@ -66,13 +68,13 @@ def _check(r, prec, exact=False):
     return rel, du
 
 
-@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1, 2])
 @pytest.mark.parametrize("tavg", [8, 16, 32, 64])
 def test_fused_matches_oracle_tavg(bf, orc, prec, tavg):
     _check(_run(bf, orc, prec=prec, tavg=tavg, seed=tavg), prec)
 
 
-@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1, 2])
 @pytest.mark.parametrize("wmode,amode,order,nant,nbeam", [
     (0, 0, 0, 63, 256),     # the reference's own shape: group weights, theta, NANT 63
     (0, 0, 1, 30, 256),     # production NANT 30, dsaX_split -r order
@@ -113,7 +115,7 @@ def test_stuff_modes_exact(bf, orc, stuff, tavg):
         assert rel.max() <= TOL[0]
 
 
-@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1, 2])
 def test_small_block_golden(bf, orc, prec):
     """The committed small block (64 ant x 48 ch x 1024 samples, 32 fan beams,
     fine-channel sin(theta) weights, tavg 32, fixed scale 2^-11)."""
@@ -209,6 +211,19 @@ def test_autotune_picks_a_variant(bf):
         small = bf.Plan(bf.config(nant=64, nchan=16, npkt=128, nbeam=nb))
         assert small.variant == want, nb
         small.close()
+    # FP6-split runs on the 4-wave shared schedule only
+    cfg6 = bf.config(nant=64, nchan=16, npkt=128, nbeam=64, tavg=32, precision=bf.PREC_FP6SPLIT,
+                     weight_mode=1, angle_mode=1)
+    fp6 = bf.Plan(cfg6)
+    assert fp6.variant == bf.VARIANT_SHARED_256
+    with pytest.raises(bf.BeamformerError):
+        fp6.variant = bf.VARIANT_CLASSIC_64
+    antpos, cal, theta = bf.synth_geometry(64, 16, 64, 5)
+    fp6.set_weights(antpos, cal, bf.reference_freqs(1498.75, 16), theta)
+    ms = fp6.autotune(2)
+    assert ms[bf.VARIANT_SHARED_256] > 0 and (np.delete(ms, bf.VARIANT_SHARED_256) < 0).all()
+    assert fp6.variant == bf.VARIANT_SHARED_256
+    fp6.close()
 
 
 def test_beam_power_matches_oracle(bf, orc):
@@ -269,7 +284,7 @@ def test_device_execute_matches_process_block(bf, orc):
 def test_full_block_subset_and_tone(bf, orc):
     """Full DSA block (64 ant x 384 ch x 4096 samples, 256 beams, tavg 32):
     oracle on two groups, tone steered into beam 77 peaks there."""
-    for prec in (0, 1):
+    for prec in (0, 1, 2):
         r = _run(bf, orc, nchan=384, npkt=2048, nbeam=256, tavg=32, prec=prec, tone=77, g=(20, 22),
                  seed=5)
         _check(r, prec)
@@ -400,7 +415,8 @@ def test_batch_graph_matches_per_block(bf):
     shapes."""
     shapes = [  # nant, nchan, npkt, nbeam, tavg, prec
         (64, 16, 128, 64, 8, 0), (64, 48, 64, 128, 32, 0), (40, 24, 256, 256, 32, 1),
-        (64, 8, 128, 512, 16, 1), (100, 16, 64, 100, 32, 0), (64, 16, 128, 64, 8, 0)]
+        (64, 8, 128, 512, 16, 1), (100, 16, 64, 100, 32, 0), (64, 16, 128, 64, 8, 0),
+        (48, 16, 128, 256, 32, 2)]
     plans, bufs, want = [], [], []
     for i, (na, nc, npk, nb, ta, pr) in enumerate(shapes):
         cfg = bf.config(nant=na, nchan=nc, npkt=npk, nbeam=nb, tavg=ta, precision=pr,

@@ -59,7 +59,7 @@ def test_config_defaults_match_reference(bf):
 
 @pytest.mark.parametrize("field,value", [
     ("nant", 0), ("nant", 257), ("nchan", 12), ("npkt", 48), ("nbeam", 0), ("nbeam", 1025),
-    ("tavg", 4), ("tavg", 24), ("precision", 2), ("weight_mode", 3), ("angle_mode", -1),
+    ("tavg", 4), ("tavg", 24), ("precision", 3), ("weight_mode", 3), ("angle_mode", -1),
     ("stuffants", 3), ("input_order", 2)])
 def test_create_rejects_bad_config(bf, field, value):
     cfg = bf.config(nant=16, nchan=16, npkt=64, nbeam=32)
@@ -67,6 +67,15 @@ def test_create_rejects_bad_config(bf, field, value):
     with pytest.raises(bf.BeamformerError) as e:
         bf.Plan(cfg)
     assert e.value.code == -1
+
+
+def test_fp6_split_is_limited_to_64_antennas(bf):
+    """FP6-split runs only on the 64-antenna shared schedule: a larger array
+    is a configuration error before any device is touched."""
+    cfg = bf.config(nant=100, nchan=16, npkt=64, nbeam=32, precision=bf.PREC_FP6SPLIT)
+    with pytest.raises(bf.BeamformerError) as e:
+        bf.Plan(cfg)
+    assert e.value.code == -1 and "FP6" in str(e.value)
 
 
 def test_create_without_gpu_fails_cleanly(bf):
