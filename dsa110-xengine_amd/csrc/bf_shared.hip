@@ -18,7 +18,7 @@
 // and ~8 (FP8) instructions per MFMA.
 //
 // FP6-split (E2M3 voltage codes v/8 and E2M3 hi + lo weights, the lo part
-// 32x larger and scaled back by the MFMA's E8M0 operand scale) runs the FP8
+// 16x larger and scaled back by the MFMA's E8M0 operand scale) runs the FP8
 // chains at twice the matrix rate: a 32x32x64 MFMA on 6-bit operands takes 32
 // cycles instead of 64.
 //
@@ -200,12 +200,12 @@ __device__ __forceinline__ void unit16(Acc<BTW> &A, WFrag<0, BTW, 1> &W, const u
 // right after its last MFMA (the rolling prefetch of unit16).  Scheduling
 // barriers keep this interleave as written.
 // One 32x32x64 MFMA of the FP8 (PREC 1) or FP6 (PREC 2) chains; LO marks the
-// weights' lo part, which FP6 stores 2^5 larger (E8M0 scales: A 2^0, B 2^-5).
+// weights' lo part, which FP6 stores 2^4 larger (E8M0 scales: A 2^0, B 2^-4).
 template <int PREC, bool LO>
 __device__ __forceinline__ f16v mfma64(const i8v &a, const i8v &b, const f16v &c) {
   if constexpr (PREC == 1) return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, 0, 0, 0);
   else if constexpr (!LO) return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 2, 2, 0, 0, 0, 0);
-  else return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 2, 2, 0, 127, 0, 122);
+  else return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 2, 2, 0, 127, 0, 123);
 }
 
 template <int PREC, int BT, bool PF, int NP>

@@ -28,11 +28,12 @@ extern "C" size_t bfk_wscale_bytes(const bfk_dims *d) {
 }
 
 // One thread per (weight channel wc, pol, beam).  The scale puts the largest
-// component of the beam's weights in [2^14, 2^15) (FP16), [128, 256) (FP8) or
-// [2, 4) (FP6); wscale holds the power factor that undoes it and the voltage
-// code's unit (2^-9 for FP8, 1/8 for FP6).  The FP6 lo part is stored 32x
-// larger (its residual is at most 1/8 there) and the MFMA's E8M0 operand
-// scale of 2^-5 takes that back.
+// component of the beam's weights in [2^14, 2^15) (FP16) or [128, 256) (FP8),
+// or at exactly 7.5, the top of E2M3 (FP6: with only two exponent bits the
+// whole range is needed, or small components lose their precision); wscale
+// holds the power factor that undoes it and the voltage code's unit (2^-9 for
+// FP8, 1/8 for FP6).  The FP6 lo part is stored 16x larger (its residual is at
+// most 1/4 there) and the MFMA's E8M0 operand scale of 2^-4 takes that back.
 __global__ void bf_weights_kernel(bfk_dims d, const float *__restrict__ antpos,
                                   const float *__restrict__ cal,
                                   const double *__restrict__ freqs,
@@ -63,11 +64,12 @@ __global__ void bf_weights_kernel(bfk_dims d, const float *__restrict__ antpos,
     }
   int k = 0;
   if (mx > 0.0) frexp(mx, &k);
-  int e = (d.prec == 0 ? 15 : d.prec == 1 ? 8 : 2) - k;
+  int e = (d.prec == 0 ? 15 : 8) - k;
   double sc = ldexp(1.0, e);
+  if (d.prec == 2) sc = mx > 0.0 ? 7.5 / mx : 1.0;
   wscale[idx] = (float)(d.prec == 0   ? ldexp(1.0, -2 * e)
                         : d.prec == 1 ? ldexp(1.0, 18 - 2 * e)
-                                      : ldexp(1.0, 6 - 2 * e));
+                                      : 64.0 / (sc * sc));
 
   int bt = b >> 5, n = b & 31;
   for (int kc = 0; kc < d.nkc; kc++)
@@ -102,7 +104,7 @@ __global__ void bf_weights_kernel(bfk_dims d, const float *__restrict__ antpos,
           for (int j = 0; j < 32; j++) {    // slot j: Wr (j < 16) or Wi of antenna j % 16
             const double w = j < 16 ? wr[16 * s2 + j] : wi[16 * s2 + j - 16];
             const uint32_t ch = e2m3_encode(w);
-            const uint32_t cl = e2m3_encode((w - e2m3_decode(ch)) * 32.0);
+            const uint32_t cl = e2m3_encode((w - e2m3_decode(ch)) * 16.0);
             const int bit = 6 * j, dw = bit >> 5, sh = bit & 31;
             hi[dw] |= ch << sh;
             lo[dw] |= cl << sh;
