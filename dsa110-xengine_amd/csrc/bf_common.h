@@ -35,8 +35,8 @@ int bfk_check(hipError_t e, const char *what);
 //   16..31 = Wi; hi and lo records at the FP8 indices, 24 of their 32 bytes
 //   used.
 // All formats use 128 bytes per lane per (cq, beam tile, antenna chunk).
-// The matching A (voltage) slots are built by unpack_f16 / nib_e4m3 /
-// nib_e2m3 + pack_e2m3.
+// The matching A (voltage) slots are built by unpack_f16 / nib_e4m3 / the
+// FP6 converter (unpack_cq_f6).
 // ---------------------------------------------------------------------------
 constexpr int WFRAG_LANE_BYTES = 128;
 
@@ -98,6 +98,7 @@ __device__ __forceinline__ uint32_t nib_e4m3(uint32_t xs) {
 // OCP FP6 E2M3 (gfx950 "fp6"): bias 1, max 7.5, subnormal step 1/8.  Round
 // to nearest even, saturating.  Code c in 0..8 encodes c/8, so a 4-bit
 // voltage v is the sign-magnitude code (v<0)<<5 | |v| worth v/8, exactly.
+// Slot j of a packed operand is at bits 6j..6j+5 of its six dwords.
 __device__ static inline uint32_t e2m3_encode(double v) {
   uint32_t sgn = v < 0 ? 0x20u : 0u;
   double a = fabs(v);
@@ -118,29 +119,6 @@ __device__ static inline double e2m3_decode(uint32_t c) {
   return s * (1.0 + m / 8.0) * ldexp(1.0, (int)e - 1);
 }
 
-// The low nibbles of 4 bytes -> 4 E2M3 codes, one per byte (as nib_e4m3,
-// with the sign at bit 5).
-__device__ __forceinline__ uint32_t nib_e2m3(uint32_t xs) {
-  const uint32_t s = xs & 0x0F0F0F0Fu;
-  const uint32_t m = xs & 0x08080808u;
-  const uint32_t t = m >> 3;
-  const uint32_t sm = (m << 1) - t;
-  return ((s ^ sm) + t) | (m << 2);
-}
-
-// 16 six-bit codes, 4 per dword in the low 6 bits of each byte (c[k] byte i
-// = slot 4k + i), packed densely into 3 dwords (slot j at bits 6j..6j+5).
-__device__ __forceinline__ void pack_e2m3(const uint32_t (&c)[4], uint32_t &d0, uint32_t &d1, uint32_t &d2) {
-  uint32_t g[4];
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const uint32_t p = ((c[k] >> 2) & 0x0FC00FC0u) | (c[k] & 0x003F003Fu);   // 12-bit pairs per half
-    g[k] = (p & 0xFFFu) | ((p >> 4) & 0xFFF000u);                            // 24 bits
-  }
-  d0 = g[0] | (g[1] << 24);
-  d1 = (g[1] >> 8) | (g[2] << 16);
-  d2 = (g[2] >> 16) | (g[3] << 8);
-}
 // Sign bits of 16 packed E2M3 slots (bits 6j+5) in each of the 3 dwords.
 constexpr uint32_t E2M3_SIGN0 = 0x20820820u, E2M3_SIGN1 = 0x08208208u, E2M3_SIGN2 = 0x82082082u;
 

@@ -111,14 +111,31 @@ __device__ __forceinline__ void unpack_cq_f6(const uint8_t *stage, uint8_t *abuf
   const uint2 x0 = *(const uint2 *)(stage + srow(r, h * 4 + 2 * q));
   const uint2 x1 = *(const uint2 *)(stage + srow(r, h * 4 + 2 * q + 1));
   const uint32_t x[4] = {x0.x, x0.y, x1.x, x1.y};
-  uint32_t cr[4], ci[4], o[6];
+  // nibble n -> half 128 + (n^8)/8 - 129 = n/8 (exact), then the hardware
+  // packs the 32 halves (Dr of the 16 antennas, then Di) into E2M3 slots
+  typedef _Float16 h32v __attribute__((ext_vector_type(32)));
+  typedef int i6v __attribute__((ext_vector_type(6)));
+  const h2v off = {(_Float16)-129.0f, (_Float16)-129.0f};
+  h2v pr[8], pi[8];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    cr[k] = nib_e2m3(x[k]);
-    ci[k] = nib_e2m3(x[k] >> 4);
+    const uint32_t x8 = x[k] ^ 0x88888888u;
+    const uint32_t lo = x8 & 0x0F0F0F0Fu, hi = (x8 >> 4) & 0x0F0F0F0Fu;
+    pr[2 * k] = __builtin_bit_cast(h2v, __builtin_amdgcn_perm(0x58585858u, lo, 0x04010400u)) + off;
+    pr[2 * k + 1] = __builtin_bit_cast(h2v, __builtin_amdgcn_perm(0x58585858u, lo, 0x04030402u)) + off;
+    pi[2 * k] = __builtin_bit_cast(h2v, __builtin_amdgcn_perm(0x58585858u, hi, 0x04010400u)) + off;
+    pi[2 * k + 1] = __builtin_bit_cast(h2v, __builtin_amdgcn_perm(0x58585858u, hi, 0x04030402u)) + off;
   }
-  pack_e2m3(cr, o[0], o[1], o[2]);
-  pack_e2m3(ci, o[3], o[4], o[5]);
+  h32v src;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    src[2 * j] = pr[j][0];
+    src[2 * j + 1] = pr[j][1];
+    src[16 + 2 * j] = pi[j][0];
+    src[16 + 2 * j + 1] = pi[j][1];
+  }
+  const i6v c = __builtin_amdgcn_cvt_scalef32_pk32_fp6_f16(src, 1.0f);
+  uint32_t o[6] = {(uint32_t)c[0], (uint32_t)c[1], (uint32_t)c[2], (uint32_t)c[3], (uint32_t)c[4], (uint32_t)c[5]};
   o[3] ^= E2M3_SIGN0;   // -Di
   o[4] ^= E2M3_SIGN1;
   o[5] ^= E2M3_SIGN2;
