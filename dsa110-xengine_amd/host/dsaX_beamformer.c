@@ -103,9 +103,14 @@ int main(int argc, char *argv[]) {
       case 'n': nant = atoi(optarg); break;
       case 'T': tavg = atoi(optarg); break;
       case 'P':
-        prec = strcmp(optarg, "fp8split") == 0   ? DSAXBF_PREC_FP8SPLIT
-               : strcmp(optarg, "fp6split") == 0 ? DSAXBF_PREC_FP6SPLIT
-                                                 : DSAXBF_PREC_FP16;
+        if (strcmp(optarg, "fp16") == 0) prec = DSAXBF_PREC_FP16;
+        else if (strcmp(optarg, "fp8split") == 0) prec = DSAXBF_PREC_FP8SPLIT;
+        else if (strcmp(optarg, "fp6split") == 0) prec = DSAXBF_PREC_FP6SPLIT;
+        else {
+          fprintf(stderr, "dsaX_beamformer: unknown precision %s\n", optarg);
+          usage();
+          return EXIT_FAILURE;
+        }
         break;
       case 'W': wmode = strcmp(optarg, "fine") == 0 ? DSAXBF_WEIGHTS_FINE : DSAXBF_WEIGHTS_GROUP; break;
       case 'A': amode = strcmp(optarg, "sin") == 0 ? DSAXBF_ANGLE_SIN : DSAXBF_ANGLE_THETA; break;

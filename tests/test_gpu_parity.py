@@ -328,11 +328,13 @@ def _dada_header():
     return h + b"\0" * (4096 - len(h))
 
 
-def test_cli_end_to_end(bf, orc, tmp_path):
+@pytest.mark.parametrize("pname,prec", [("fp16", 0), ("fp8split", 1), ("fp6split", 2)])
+def test_cli_end_to_end(bf, orc, tmp_path, pname, prec):
     """dsaX_beamformer file mode: header passthrough, block 0 zeroed, AGC
-    scale from block k applied to block k+1, short final block ends the run."""
+    scale from block k applied to block k+1, short final block ends the run;
+    -P selects the precision."""
     nant, npkt = 24, 2048
-    cfg = bf.config(nant=nant, nchan=384, npkt=npkt, nbeam=256, tavg=8)
+    cfg = bf.config(nant=nant, nchan=384, npkt=npkt, nbeam=256, tavg=8, precision=prec)
     rng = np.random.default_rng(3)
     antpos = rng.uniform(-600, 600, 64).astype(np.float32)
     calw = (rng.normal(size=(64, 48, 2, 2))).astype(np.float32)
@@ -349,8 +351,8 @@ def test_cli_end_to_end(bf, orc, tmp_path):
         fh.write(blocks[0][: bf.input_bytes(cfg) // 2].tobytes())   # short final block
     fout = tmp_path / "out.dada"
     exe = os.path.join(ROOT, "dsa110-xengine_amd", "bin", "dsaX_beamformer")
-    r = subprocess.run([exe, "-f", str(wfile), "-z", "1498.75", "-n", str(nant), "-I", str(fin),
-                        "-O", str(fout)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "-f", str(wfile), "-z", "1498.75", "-n", str(nant), "-P", pname,
+                        "-I", str(fin), "-O", str(fout)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     data = open(fout, "rb").read()
     assert data[:4096] == _dada_header()

@@ -207,3 +207,6 @@ def test_cli_usage_and_missing_io():
     assert r.returncode == 0 and "-z fch1 in MHz [default 1530]" in r.stdout
     r = subprocess.run([exe, "-f", "nofile"], capture_output=True, text=True)
     assert r.returncode != 0 and "psrdada" in r.stderr
+    assert "fp16|fp8split|fp6split" in subprocess.run([exe, "-h"], capture_output=True, text=True).stdout
+    r = subprocess.run([exe, "-P", "fp4", "-I", "-", "-O", "-"], capture_output=True, text=True)
+    assert r.returncode != 0 and "unknown precision fp4" in r.stderr
