@@ -252,6 +252,21 @@ def stream_bench(dsaxbf, ev, nblocks=8, seed=77):
         plan.execute(d_in, d_sc, d_out, None, None, stream)
     ev.record(k1, stream)
     kern_ms = ev.ms(k0, k1) / 3
+    # the same block on the FP6-split path (kernel alone, device-resident data)
+    cfg6 = dsaxbf.config(nant=na, nchan=nc, npkt=npk, nbeam=nb, tavg=32,
+                         precision=dsaxbf.PREC_FP6SPLIT, weight_mode=dsaxbf.WEIGHTS_FINE,
+                         angle_mode=dsaxbf.ANGLE_SIN)
+    plan6 = dsaxbf.Plan(cfg6)
+    plan6.set_weights(antpos, cal, freqs, theta)
+    plan6.execute(d_in, d_sc, d_out, None, None, stream)
+    ev.sync()
+    k0, k1 = ev.create(2)
+    ev.record(k0, stream)
+    for _ in range(3):
+        plan6.execute(d_in, d_sc, d_out, None, None, stream)
+    ev.record(k1, stream)
+    kern6_ms = ev.ms(k0, k1) / 3
+    plan6.close()
     hu0, hu1 = ev.create(2)
     ev.record(hu0, stream)
     d_in.upload(pipe.input(1), stream)
@@ -284,6 +299,7 @@ def stream_bench(dsaxbf, ev, nblocks=8, seed=77):
         "h2d_ms": h2d_ms, "h2d_GB_s": nin / (h2d_ms / 1e3) / 1e9,
         "kernel_ms": kern_ms, "kernel_tflops": flop / (kern_ms / 1e3) / 1e12,
         "kernel_mfma_frac_fp8": flop / (kern_ms / 1e3) / 1e12 / PEAK_TFLOPS["fp8split"],
+        "kernel_ms_fp6split": kern6_ms,
         "kernel_algorithmic_GB_s": (nin + wbytes) / (kern_ms / 1e3) / 1e9,
         "first_two_blocks_s": first_two_s,
     }

@@ -383,6 +383,58 @@ template <int NB, int NK> struct WFrag<2, NB, NK> {
   }
 };
 
+// One 32x32x64 MFMA of the FP8 (PREC 1) or FP6 (PREC 2) chains; LO marks the
+// weights' lo part, which FP6 stores 2^4 larger (E8M0 scales: A 2^0, B 2^-4).
+template <int PREC, bool LO>
+__device__ __forceinline__ f16v mfma64(const i8v &a, const i8v &b, const f16v &c) {
+  if constexpr (PREC == 1) return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, 0, 0, 0);
+  else if constexpr (!LO) return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 2, 2, 0, 0, 0, 0);
+  else return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 2, 2, 0, 127, 0, 123);
+}
+
+// FP6 A_re of 16 antennas of one sample row (4 packed dwords, byte k of x[i]
+// = antenna 4i + k) as six dwords: slots 0..15 = Dr, 16..31 = -Di.  A byte
+// permute makes each nibble n the exact half 128 + (n^8)/8, minus 129 = n/8,
+// and the converter packs the 32 halves.
+__device__ __forceinline__ void f6_operand(const uint32_t (&x)[4], uint32_t (&o)[6]) {
+  typedef _Float16 h32v __attribute__((ext_vector_type(32)));
+  typedef int i6v __attribute__((ext_vector_type(6)));
+  const h2v off = {(_Float16)-129.0f, (_Float16)-129.0f};
+  h2v pr[8], pi[8];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t x8 = x[k] ^ 0x88888888u;
+    const uint32_t lo = x8 & 0x0F0F0F0Fu, hi = (x8 >> 4) & 0x0F0F0F0Fu;
+    pr[2 * k] = __builtin_bit_cast(h2v, __builtin_amdgcn_perm(0x58585858u, lo, 0x04010400u)) + off;
+    pr[2 * k + 1] = __builtin_bit_cast(h2v, __builtin_amdgcn_perm(0x58585858u, lo, 0x04030402u)) + off;
+    pi[2 * k] = __builtin_bit_cast(h2v, __builtin_amdgcn_perm(0x58585858u, hi, 0x04010400u)) + off;
+    pi[2 * k + 1] = __builtin_bit_cast(h2v, __builtin_amdgcn_perm(0x58585858u, hi, 0x04030402u)) + off;
+  }
+  h32v src;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    src[2 * j] = pr[j][0];
+    src[2 * j + 1] = pr[j][1];
+    src[16 + 2 * j] = pi[j][0];
+    src[16 + 2 * j + 1] = pi[j][1];
+  }
+  const i6v c = __builtin_amdgcn_cvt_scalef32_pk32_fp6_f16(src, 1.0f);
+  o[0] = (uint32_t)c[0];
+  o[1] = (uint32_t)c[1];
+  o[2] = (uint32_t)c[2];
+  o[3] = (uint32_t)c[3] ^ E2M3_SIGN0;   // -Di
+  o[4] = (uint32_t)c[4] ^ E2M3_SIGN1;
+  o[5] = (uint32_t)c[5] ^ E2M3_SIGN2;
+}
+
+// A_im = [Di | Dr] from A_re = [Dr | -Di] (16 six-bit slots each; dwords 6
+// and 7 of the MFMA operand are unused).
+__device__ __forceinline__ void f6_re_im(const uint4 &r0, const uint2 &r1, i8v &are, i8v &aim) {
+  are = (i8v){(int)r0.x, (int)r0.y, (int)r0.z, (int)r0.w, (int)r1.x, (int)r1.y, 0, 0};
+  aim = (i8v){(int)(r0.w ^ E2M3_SIGN0), (int)(r1.x ^ E2M3_SIGN1), (int)(r1.y ^ E2M3_SIGN2),
+              (int)r0.x, (int)r0.y, (int)r0.z, 0, 0};
+}
+
 // One GEMM unit: 32 staged sample rows of one (channel, pol) against the
 // wave's weights.  Rows are rowb bytes apart; lane (n, h) reads row n,
 // antennas 64kc + 32h .. 64kc + 32h + 31 for each chunk kc.

@@ -275,7 +275,7 @@ extern "C" int bfk_classic_beams(int variant, int nkc);
 
 extern "C" int bfk_variant_supported(const bfk_dims *d, int v) {
   if (d->nant_pad != 64 * d->nkc || (d->nkc != 1 && d->nkc != 2 && d->nkc != 4)) return 0;
-  if (d->prec == 2) return v == BFK_VAR_SHARED_256 && d->nkc == 1;   // FP6: 4-wave shared tiles only
+  if (d->prec == 2) return v == BFK_VAR_SHARED_256;   // FP6: shared tiles only
   if (v == BFK_VAR_PIPELINED_256) return d->nkc == 1;
   if (v == BFK_VAR_SHARED_256) return 1;       // 4-wave tiles to 64 antennas, 8-wave above
   return bfk_classic_beams(v, d->nkc) > 0;

@@ -111,34 +111,8 @@ __device__ __forceinline__ void unpack_cq_f6(const uint8_t *stage, uint8_t *abuf
   const uint2 x0 = *(const uint2 *)(stage + srow(r, h * 4 + 2 * q));
   const uint2 x1 = *(const uint2 *)(stage + srow(r, h * 4 + 2 * q + 1));
   const uint32_t x[4] = {x0.x, x0.y, x1.x, x1.y};
-  // nibble n -> half 128 + (n^8)/8 - 129 = n/8 (exact), then the hardware
-  // packs the 32 halves (Dr of the 16 antennas, then Di) into E2M3 slots
-  typedef _Float16 h32v __attribute__((ext_vector_type(32)));
-  typedef int i6v __attribute__((ext_vector_type(6)));
-  const h2v off = {(_Float16)-129.0f, (_Float16)-129.0f};
-  h2v pr[8], pi[8];
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const uint32_t x8 = x[k] ^ 0x88888888u;
-    const uint32_t lo = x8 & 0x0F0F0F0Fu, hi = (x8 >> 4) & 0x0F0F0F0Fu;
-    pr[2 * k] = __builtin_bit_cast(h2v, __builtin_amdgcn_perm(0x58585858u, lo, 0x04010400u)) + off;
-    pr[2 * k + 1] = __builtin_bit_cast(h2v, __builtin_amdgcn_perm(0x58585858u, lo, 0x04030402u)) + off;
-    pi[2 * k] = __builtin_bit_cast(h2v, __builtin_amdgcn_perm(0x58585858u, hi, 0x04010400u)) + off;
-    pi[2 * k + 1] = __builtin_bit_cast(h2v, __builtin_amdgcn_perm(0x58585858u, hi, 0x04030402u)) + off;
-  }
-  h32v src;
-#pragma unroll
-  for (int j = 0; j < 8; j++) {
-    src[2 * j] = pr[j][0];
-    src[2 * j + 1] = pr[j][1];
-    src[16 + 2 * j] = pi[j][0];
-    src[16 + 2 * j + 1] = pi[j][1];
-  }
-  const i6v c = __builtin_amdgcn_cvt_scalef32_pk32_fp6_f16(src, 1.0f);
-  uint32_t o[6] = {(uint32_t)c[0], (uint32_t)c[1], (uint32_t)c[2], (uint32_t)c[3], (uint32_t)c[4], (uint32_t)c[5]};
-  o[3] ^= E2M3_SIGN0;   // -Di
-  o[4] ^= E2M3_SIGN1;
-  o[5] ^= E2M3_SIGN2;
+  uint32_t o[6];
+  f6_operand(x, o);
   uint8_t *p = abuf + tt * 4096 + q * 1536;
   *(uint4 *)(p + li * 16) = make_uint4(o[0], o[1], o[2], o[3]);
   *(uint2 *)(p + 1024 + li * 8) = make_uint2(o[4], o[5]);
@@ -216,15 +190,6 @@ __device__ __forceinline__ void unit16(Acc<BTW> &A, WFrag<0, BTW, 1> &W, const u
 // elements per MFMA.  With PF, each K step of W[BT] is reloaded from row wnext
 // right after its last MFMA (the rolling prefetch of unit16).  Scheduling
 // barriers keep this interleave as written.
-// One 32x32x64 MFMA of the FP8 (PREC 1) or FP6 (PREC 2) chains; LO marks the
-// weights' lo part, which FP6 stores 2^4 larger (E8M0 scales: A 2^0, B 2^-4).
-template <int PREC, bool LO>
-__device__ __forceinline__ f16v mfma64(const i8v &a, const i8v &b, const f16v &c) {
-  if constexpr (PREC == 1) return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, 0, 0, 0);
-  else if constexpr (!LO) return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 2, 2, 0, 0, 0, 0);
-  else return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 2, 2, 0, 127, 0, 123);
-}
-
 template <int PREC, int BT, bool PF, int NP>
 __device__ __forceinline__ void chain8(f16v &cr, f16v &ci, WFrag<PREC, BTW, 1> &W, const i8v (&are)[2],
                                        const i8v (&aim)[2], int lane, const uint8_t *wfrag, long wnext,
@@ -279,10 +244,7 @@ __device__ __forceinline__ void unit8(Pending &pd, WFrag<PREC, BTW, 1> &W, const
     } else {
       const uint4 r0 = *(const uint4 *)(unit + s2 * 1536 + lane * 16);
       const uint2 r1 = *(const uint2 *)(unit + s2 * 1536 + 1024 + lane * 8);
-      // A_re = [Dr | -Di] (16 slots each) -> A_im = [Di | Dr]; dwords 6, 7 unused
-      are[s2] = (i8v){(int)r0.x, (int)r0.y, (int)r0.z, (int)r0.w, (int)r1.x, (int)r1.y, 0, 0};
-      aim[s2] = (i8v){(int)(r0.w ^ E2M3_SIGN0), (int)(r1.x ^ E2M3_SIGN1), (int)(r1.y ^ E2M3_SIGN2),
-                      (int)r0.x, (int)r0.y, (int)r0.z, 0, 0};
+      f6_re_im(r0, r1, are[s2], aim[s2]);
     }
   }
   chain8<PREC, 0, PF, NP>(cr, ci, W, are, aim, lane, wfrag, wnext, pd, into0);

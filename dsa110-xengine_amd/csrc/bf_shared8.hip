@@ -158,6 +158,25 @@ __device__ __forceinline__ void unpack8_f8(const uint8_t *stage, uint8_t *abuf, 
   }
 }
 
+// FP6 A_re, the record layout of unpack8_f8: six dwords per lane, dwords 0..3
+// in plane 0 and 4..5 in the first half of plane 1's 16-byte lane slot.
+template <int NKC>
+__device__ __forceinline__ void unpack8_f6(const uint8_t *stage, uint8_t *abuf, int cql, int tid) {
+  const int li = tid & 63, n = li & 31, h = li >> 5, q = tid >> 6;
+#pragma unroll
+  for (int i = 0; i < NKC / 2; i++) {
+    const int C = q * (NKC / 2) + i, tt = C / (2 * NKC), kc = (C >> 1) % NKC, s2 = C & 1;
+    const int r = cql * TCU + tt * 32 + n;
+    const uint4 v = *(const uint4 *)(stage + srow8<NKC>(r, kc * 4 + h * 2 + s2));
+    const uint32_t x[4] = {v.x, v.y, v.z, v.w};
+    uint32_t o[6];
+    f6_operand(x, o);
+    uint8_t *p = abuf + ((C * 2) * 64 + li) * 16;
+    *(uint4 *)p = make_uint4(o[0], o[1], o[2], o[3]);
+    *(uint2 *)(p + 64 * 16) = make_uint2(o[4], o[5]);
+  }
+}
+
 // One unit from LDS A_re: gemm_unit's MFMA chains (one beam tile), A_im
 // derived.  With PF, each K step of W is reloaded from row wnext right after
 // its last MFMA has issued (rolling prefetch, pinned by scheduling barriers
@@ -188,15 +207,20 @@ __device__ __forceinline__ void unit8w(Acc<1> &A, WFrag<PREC, 1, NKC> &W, const 
 #pragma unroll
       for (int s2 = 0; s2 < 2; s2++) {
         const uint8_t *src = unit + (((kc * 2 + s2) * 2) * 64 + lane) * 16;
-        const uint4 r0 = *(const uint4 *)src, r1 = *(const uint4 *)(src + 64 * 16);
-        // A_re dwords [Dr, -Di] per 4 antennas -> A_im [Di, Dr]
-        const i8v are = {(int)r0.x, (int)r0.y, (int)r0.z, (int)r0.w, (int)r1.x, (int)r1.y, (int)r1.z, (int)r1.w};
-        const i8v aim = {(int)(r0.y ^ 0x80808080u), (int)r0.x, (int)(r0.w ^ 0x80808080u), (int)r0.z,
-                         (int)(r1.y ^ 0x80808080u), (int)r1.x, (int)(r1.w ^ 0x80808080u), (int)r1.z};
-        A.r[0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.hi[0][kc][s2], A.r[0], 0, 0, 0, 0, 0, 0);
-        A.i[0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.hi[0][kc][s2], A.i[0], 0, 0, 0, 0, 0, 0);
-        A.r[0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(are, W.lo[0][kc][s2], A.r[0], 0, 0, 0, 0, 0, 0);
-        A.i[0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aim, W.lo[0][kc][s2], A.i[0], 0, 0, 0, 0, 0, 0);
+        i8v are, aim;
+        if constexpr (PREC == 1) {
+          const uint4 r0 = *(const uint4 *)src, r1 = *(const uint4 *)(src + 64 * 16);
+          // A_re dwords [Dr, -Di] per 4 antennas -> A_im [Di, Dr]
+          are = (i8v){(int)r0.x, (int)r0.y, (int)r0.z, (int)r0.w, (int)r1.x, (int)r1.y, (int)r1.z, (int)r1.w};
+          aim = (i8v){(int)(r0.y ^ 0x80808080u), (int)r0.x, (int)(r0.w ^ 0x80808080u), (int)r0.z,
+                      (int)(r1.y ^ 0x80808080u), (int)r1.x, (int)(r1.w ^ 0x80808080u), (int)r1.z};
+        } else {
+          f6_re_im(*(const uint4 *)src, *(const uint2 *)(src + 64 * 16), are, aim);
+        }
+        A.r[0] = mfma64<PREC, false>(are, W.hi[0][kc][s2], A.r[0]);
+        A.i[0] = mfma64<PREC, false>(aim, W.hi[0][kc][s2], A.i[0]);
+        A.r[0] = mfma64<PREC, true>(are, W.lo[0][kc][s2], A.r[0]);
+        A.i[0] = mfma64<PREC, true>(aim, W.lo[0][kc][s2], A.i[0]);
         if constexpr (PF) {
           __builtin_amdgcn_sched_barrier(0x386);
           W.load_k(wfrag, wnext, 0, kc, s2, lane);
@@ -222,9 +246,9 @@ __device__ __forceinline__ void shared8_tile(const BfArgs &a, int L, uint8_t *st
   const int n = lane & 31, h = lane >> 5;
   const int btw = t.bset * NW8 + wave;       // this wave's 32-beam tile
   // Next sub-step's voltages held in registers during the current one: only
-  // where that fits beside the weights (128 antennas, FP8); elsewhere each
-  // sub-step loads its own after the previous one.
-  constexpr bool PFS = PREC == 1 && NKC == 2;
+  // where that fits beside the weights (128 antennas, FP8 and FP6); elsewhere
+  // each sub-step loads its own after the previous one.
+  constexpr bool PFS = PREC >= 1 && NKC == 2;
 
   // sub-step 0's voltages and the first weights; each later sub-step's
   // voltages are loaded while the previous one is computed
@@ -249,7 +273,8 @@ __device__ __forceinline__ void shared8_tile(const BfArgs &a, int L, uint8_t *st
   for (int sub = 0; sub < NKC; sub++) {
     // this sub-step's stage is in LDS; its first A_re operands:
     if constexpr (PREC == 0) unpack8_f16<NKC>(stage, abuf, 0, tid);
-    else unpack8_f8<NKC>(stage, abuf, 0, tid);
+    else if constexpr (PREC == 1) unpack8_f8<NKC>(stage, abuf, 0, tid);
+    else unpack8_f6<NKC>(stage, abuf, 0, tid);
     __syncthreads();
     for (int cql = 0; cql < S::CQS; cql++) {
       const int cq = sub * S::CQS + cql;
@@ -266,7 +291,11 @@ __device__ __forceinline__ void shared8_tile(const BfArgs &a, int L, uint8_t *st
         detect<TAVG>(A, W.sc, part[1]);
       } else {
         const uint8_t *ab = abuf + (cql & 1) * 2 * S::UNIT8;
-        if (!last) unpack8_f8<NKC>(stage, abuf + ((cql + 1) & 1) * 2 * S::UNIT8, cql + 1, tid);
+        if (!last) {
+          uint8_t *nb = abuf + ((cql + 1) & 1) * 2 * S::UNIT8;
+          if constexpr (PREC == 1) unpack8_f8<NKC>(stage, nb, cql + 1, tid);
+          else unpack8_f6<NKC>(stage, nb, cql + 1, tid);
+        }
         Acc<1> A;
         unit8w<PREC, NKC, false>(A, W, ab, lane, a.wfrag, 0);
         detect<TAVG>(A, W.sc, part[0]);
@@ -356,10 +385,12 @@ extern "C" int bfk_beamform_shared8(const bfk_dims *d, const uint8_t *d_in, cons
   hipStream_t s = (hipStream_t)stream;
   if (d->nkc == 2) {
     if (d->prec == 0) launch8<0, 2>(d->tavg, grid, s, a);
-    else launch8<1, 2>(d->tavg, grid, s, a);
+    else if (d->prec == 1) launch8<1, 2>(d->tavg, grid, s, a);
+    else launch8<2, 2>(d->tavg, grid, s, a);
   } else {
     if (d->prec == 0) launch8<0, 4>(d->tavg, grid, s, a);
-    else launch8<1, 4>(d->tavg, grid, s, a);
+    else if (d->prec == 1) launch8<1, 4>(d->tavg, grid, s, a);
+    else launch8<2, 4>(d->tavg, grid, s, a);
   }
   return bfk_check(hipGetLastError(), "bf_shared8_kernel");
 }
@@ -379,10 +410,12 @@ extern "C" int bfk_beamform_shared8_batch(int prec, int tavg, int nkc, const voi
   const BfArgs *jobs = (const BfArgs *)d_jobs;
   if (nkc == 2) {
     if (prec == 0) launch8_batch<0, 2>(tavg, grid, s, jobs, d_tile_start, njobs);
-    else launch8_batch<1, 2>(tavg, grid, s, jobs, d_tile_start, njobs);
+    else if (prec == 1) launch8_batch<1, 2>(tavg, grid, s, jobs, d_tile_start, njobs);
+    else launch8_batch<2, 2>(tavg, grid, s, jobs, d_tile_start, njobs);
   } else {
     if (prec == 0) launch8_batch<0, 4>(tavg, grid, s, jobs, d_tile_start, njobs);
-    else launch8_batch<1, 4>(tavg, grid, s, jobs, d_tile_start, njobs);
+    else if (prec == 1) launch8_batch<1, 4>(tavg, grid, s, jobs, d_tile_start, njobs);
+    else launch8_batch<2, 4>(tavg, grid, s, jobs, d_tile_start, njobs);
   }
   return bfk_check(hipGetLastError(), "bf_shared8_batch_kernel");
 }

@@ -73,8 +73,6 @@ int dsaxbf_check_config(const dsaxbf_config *c) {
   if (c->precision != DSAXBF_PREC_FP16 && c->precision != DSAXBF_PREC_FP8SPLIT &&
       c->precision != DSAXBF_PREC_FP6SPLIT)
     return dsaxbf_fail(DSAXBF_EINVAL, "precision=%d unknown", c->precision);
-  if (c->precision == DSAXBF_PREC_FP6SPLIT && c->nant > 64)
-    return dsaxbf_fail(DSAXBF_EINVAL, "precision FP6-split runs up to 64 antennas (nant=%d)", c->nant);
   if (c->weight_mode != 0 && c->weight_mode != 1)
     return dsaxbf_fail(DSAXBF_EINVAL, "weight_mode=%d unknown", c->weight_mode);
   if (c->angle_mode != 0 && c->angle_mode != 1)

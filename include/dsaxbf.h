@@ -64,9 +64,9 @@ enum dsaxbf_precision {
   DSAXBF_PREC_FP16    = 0,  /* FP16 weights x FP16 voltages, FP32 accumulate   */
   DSAXBF_PREC_FP8SPLIT = 1, /* FP8 E4M3 voltages x (hi+lo) FP8 weights, FP32   */
   DSAXBF_PREC_FP6SPLIT = 2  /* FP6 E2M3 voltages x (hi+lo) FP6 weights, FP32
-                               accumulate, at twice the FP8 matrix rate; up to
-                               64 antennas (shared schedule only).  No
-                               counterpart in the reference.                  */
+                               accumulate, at twice the FP8 matrix rate (shared
+                               schedule only).  No counterpart in the
+                               reference.                                     */
 };
 enum dsaxbf_weight_mode {
   DSAXBF_WEIGHTS_GROUP = 0, /* one weight per 8-channel group at freqs[8g+4]

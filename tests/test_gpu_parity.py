@@ -86,7 +86,7 @@ def test_fused_matches_oracle_shapes(bf, orc, prec, wmode, amode, order, nant, n
                 nchan=24, seed=nant + nbeam), prec)
 
 
-@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1, 2])
 @pytest.mark.parametrize("nant,nbeam,order,wmode,tavg", [
     (100, 256, 0, 1, 8),       # two 64-antenna chunks, padded
     (128, 300, 1, 0, 32),      # two full chunks, split -r order, group weights

@@ -69,15 +69,6 @@ def test_create_rejects_bad_config(bf, field, value):
     assert e.value.code == -1
 
 
-def test_fp6_split_is_limited_to_64_antennas(bf):
-    """FP6-split runs only on the 64-antenna shared schedule: a larger array
-    is a configuration error before any device is touched."""
-    cfg = bf.config(nant=100, nchan=16, npkt=64, nbeam=32, precision=bf.PREC_FP6SPLIT)
-    with pytest.raises(bf.BeamformerError) as e:
-        bf.Plan(cfg)
-    assert e.value.code == -1 and "FP6" in str(e.value)
-
-
 def test_create_without_gpu_fails_cleanly(bf):
     import torch
     if torch.cuda.is_available():
