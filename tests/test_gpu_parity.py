@@ -290,7 +290,7 @@ def test_full_block_subset_and_tone(bf, orc):
         assert np.nansum(r["power"], axis=(0, 2)).argmax() == 77
 
 
-@pytest.mark.parametrize("prec", [0, 2])
+@pytest.mark.parametrize("prec", [0, 1, 2])
 def test_reference_test_bf_scenario(bf, orc, tmp_path, prec):
     """scripts/test_bf.py: dsaX_fake replicates utils/packet.out over a
     2048-packet, NANT=63 block (dsaX_fake.c:250-267) and dsaX_beamformer runs
