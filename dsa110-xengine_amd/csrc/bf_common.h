@@ -32,8 +32,9 @@ int bfk_check(hipError_t e, const char *what);
 //   index ((((cq*nbt + bt)*nkc + kc)*2 + s2)*2 + hl)*64 + lane.
 // FP6 (the same MFMA with E2M3 operands, 6-bit slot j at bits 6j..6j+5 of six
 //   dwords): step s2 as for FP8, slots 0..15 = Wr of the 16 antennas, slots
-//   16..31 = Wi; hi and lo records at the FP8 indices, 24 of their 32 bytes
-//   used.
+//   16..31 = Wi; hi and lo records at the FP8 indices, the operand in bytes
+//   0..23; the hi record's dwords 6 and 7 hold the lane's E8M0 B-operand
+//   scales of the hi and lo parts.
 // All formats use 128 bytes per lane per (cq, beam tile, antenna chunk).
 // The matching A (voltage) slots are built by unpack_f16 / nib_e4m3 / the
 // FP6 converter (unpack_cq_f6).
@@ -276,6 +277,9 @@ template <int NB, int NK> struct WFrag<0, NB, NK> {
 template <int NB, int NK> struct WFrag<1, NB, NK> {
   i8v hi[NB][NK][2], lo[NB][NK][2];
   float sc[NB];
+  // E8M0 operand scales of the hi / lo parts: FP8 fragments have none
+  __device__ __forceinline__ int shi(int, int, int) const { return 0; }
+  __device__ __forceinline__ int slo(int, int, int) const { return 0; }
   __device__ __forceinline__ void load(const uint8_t *wfrag, const float *wscale, long wrow_bt,
                                        long sc_off, int lane, int n) {
 #pragma unroll
@@ -345,10 +349,14 @@ template <int NB, int NK> struct WFrag<1, NB, NK> {
   }
 };
 
-// FP6 hi + lo fragments: 6 dwords each (the top two of the i8v are unused).
+// FP6 hi + lo fragments: 6 operand dwords each; the hi fragment's dwords 6
+// and 7 (which the MFMA does not read) carry the lane's E8M0 scales of both
+// parts, the lo fragment's are unused.
 template <int NB, int NK> struct WFrag<2, NB, NK> {
   i8v hi[NB][NK][2], lo[NB][NK][2];
   float sc[NB];
+  __device__ __forceinline__ int shi(int bt, int kc, int s2) const { return hi[bt][kc][s2][6]; }
+  __device__ __forceinline__ int slo(int bt, int kc, int s2) const { return hi[bt][kc][s2][7]; }
   static __device__ __forceinline__ i8v ld6(const uint8_t *p) {
     typedef int i4v __attribute__((ext_vector_type(4)));
     typedef int i2v __attribute__((ext_vector_type(2)));
@@ -360,7 +368,7 @@ template <int NB, int NK> struct WFrag<2, NB, NK> {
   }
   __device__ __forceinline__ void load_k(const uint8_t *wfrag, long wrow_bt, int bt, int kc, int s2, int lane) {
     const long rec = ((((wrow_bt + bt) * NK + kc) * 2 + s2) * 2) * 64 + lane;
-    hi[bt][kc][s2] = ld6(wfrag + rec * 32);
+    hi[bt][kc][s2] = *(const i8v *)(wfrag + rec * 32);
     lo[bt][kc][s2] = ld6(wfrag + (rec + 64) * 32);
   }
   __device__ __forceinline__ void load_scale(const float *wscale, long sc_off, int n) {
@@ -383,13 +391,13 @@ template <int NB, int NK> struct WFrag<2, NB, NK> {
   }
 };
 
-// One 32x32x64 MFMA of the FP8 (PREC 1) or FP6 (PREC 2) chains; LO marks the
-// weights' lo part, which FP6 stores 2^4 larger (E8M0 scales: A 2^0, B 2^-4).
-template <int PREC, bool LO>
-__device__ __forceinline__ f16v mfma64(const i8v &a, const i8v &b, const f16v &c) {
+// One 32x32x64 MFMA of the FP8 (PREC 1) or FP6 (PREC 2) chains.  FP6 weight
+// parts are stored 2^k larger per lane; the lane's E8M0 B scale sb = 127 - k
+// takes that back (A scale 2^0).  FP8 uses no scales.
+template <int PREC>
+__device__ __forceinline__ f16v mfma64(const i8v &a, const i8v &b, const f16v &c, int sb) {
   if constexpr (PREC == 1) return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, 0, 0, 0);
-  else if constexpr (!LO) return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 2, 2, 0, 0, 0, 0);
-  else return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 2, 2, 0, 127, 0, 123);
+  else return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 2, 2, 0, 127, 0, sb);
 }
 
 // FP6 A_re of 16 antennas of one sample row (4 packed dwords, byte k of x[i]

@@ -17,8 +17,9 @@
 // swap-and-negate of A_re = [Dr, -Di]).  That brings the VALU work to ~4 (FP16)
 // and ~8 (FP8) instructions per MFMA.
 //
-// FP6-split (E2M3 voltage codes v/8 and E2M3 hi + lo weights, the lo part
-// 16x larger and scaled back by the MFMA's E8M0 operand scale) runs the FP8
+// FP6-split (E2M3 voltage codes v/8 and E2M3 hi + lo weights, each part
+// boosted per lane by a power of two that the MFMA's E8M0 operand scale takes
+// back) runs the FP8
 // chains at twice the matrix rate: a 32x32x64 MFMA on 6-bit operands takes 32
 // cycles instead of 64.
 //
@@ -202,18 +203,18 @@ __device__ __forceinline__ void chain8(f16v &cr, f16v &ci, WFrag<PREC, BTW, 1> &
 #pragma unroll
   for (int s2 = 0; s2 < 2; s2++) {
     const int k = 4 * s2;   // MFMA slot of this step's first MFMA
-    cr = mfma64<PREC, false>(are[s2], W.hi[BT][0][s2], cr);
+    cr = mfma64<PREC>(are[s2], W.hi[BT][0][s2], cr, W.shi(BT, 0, s2));
     if (k > 0) { det_elem<NP>(pd, 2 * k - 2, sd); det_elem<NP>(pd, 2 * k - 1, sd); }
     __builtin_amdgcn_sched_barrier(0);
-    ci = mfma64<PREC, false>(aim[s2], W.hi[BT][0][s2], ci);
+    ci = mfma64<PREC>(aim[s2], W.hi[BT][0][s2], ci, W.shi(BT, 0, s2));
     det_elem<NP>(pd, 2 * k, sd);
     det_elem<NP>(pd, 2 * k + 1, sd);
     __builtin_amdgcn_sched_barrier(0);
-    cr = mfma64<PREC, true>(are[s2], W.lo[BT][0][s2], cr);
+    cr = mfma64<PREC>(are[s2], W.lo[BT][0][s2], cr, W.slo(BT, 0, s2));
     det_elem<NP>(pd, 2 * k + 2, sd);
     det_elem<NP>(pd, 2 * k + 3, sd);
     __builtin_amdgcn_sched_barrier(0);
-    ci = mfma64<PREC, true>(aim[s2], W.lo[BT][0][s2], ci);
+    ci = mfma64<PREC>(aim[s2], W.lo[BT][0][s2], ci, W.slo(BT, 0, s2));
     det_elem<NP>(pd, 2 * k + 4, sd);
     det_elem<NP>(pd, 2 * k + 5, sd);
     if constexpr (PF) W.load_one(wfrag, wnext, BT, s2, lane);

@@ -217,10 +217,10 @@ __device__ __forceinline__ void unit8w(Acc<1> &A, WFrag<PREC, 1, NKC> &W, const 
         } else {
           f6_re_im(*(const uint4 *)src, *(const uint2 *)(src + 64 * 16), are, aim);
         }
-        A.r[0] = mfma64<PREC, false>(are, W.hi[0][kc][s2], A.r[0]);
-        A.i[0] = mfma64<PREC, false>(aim, W.hi[0][kc][s2], A.i[0]);
-        A.r[0] = mfma64<PREC, true>(are, W.lo[0][kc][s2], A.r[0]);
-        A.i[0] = mfma64<PREC, true>(aim, W.lo[0][kc][s2], A.i[0]);
+        A.r[0] = mfma64<PREC>(are, W.hi[0][kc][s2], A.r[0], W.shi(0, kc, s2));
+        A.i[0] = mfma64<PREC>(aim, W.hi[0][kc][s2], A.i[0], W.shi(0, kc, s2));
+        A.r[0] = mfma64<PREC>(are, W.lo[0][kc][s2], A.r[0], W.slo(0, kc, s2));
+        A.i[0] = mfma64<PREC>(aim, W.lo[0][kc][s2], A.i[0], W.slo(0, kc, s2));
         if constexpr (PF) {
           __builtin_amdgcn_sched_barrier(0x386);
           W.load_k(wfrag, wnext, 0, kc, s2, lane);

@@ -27,13 +27,23 @@ extern "C" size_t bfk_wscale_bytes(const bfk_dims *d) {
   return (size_t)d->ncal * 2 * d->nbeam_pad * sizeof(float);
 }
 
+// The largest k in 0..60 with m * 2^k <= 7.5 (0 for m = 0).
+__device__ static inline int lane_boost(double m) {
+  if (!(m > 0.0)) return 0;
+  int k = (int)floor(log2(7.5 / m));
+  k = k < 0 ? 0 : k > 60 ? 60 : k;
+  while (k > 0 && ldexp(m, k) > 7.5) k--;
+  return k;
+}
+
 // One thread per (weight channel wc, pol, beam).  The scale puts the largest
 // component of the beam's weights in [2^14, 2^15) (FP16) or [128, 256) (FP8),
-// or at exactly 7.5, the top of E2M3 (FP6: with only two exponent bits the
-// whole range is needed, or small components lose their precision); wscale
-// holds the power factor that undoes it and the voltage code's unit (2^-9 for
-// FP8, 1/8 for FP6).  The FP6 lo part is stored 16x larger (its residual is at
-// most 1/4 there) and the MFMA's E8M0 operand scale of 2^-4 takes that back.
+// or at exactly 7.5, the top of E2M3 (FP6); wscale holds the power factor
+// that undoes it and the voltage code's unit (2^-9 for FP8, 1/8 for FP6).
+// E2M3 has only two exponent bits, so FP6 also boosts each lane's 32 slots
+// (16 antennas of one beam and K step) by the largest power of two that keeps
+// them within 7.5, and its lo part by the largest that keeps the lane's
+// residuals within 7.5; the lane's E8M0 MFMA operand scales take both back.
 __global__ void bf_weights_kernel(bfk_dims d, const float *__restrict__ antpos,
                                   const float *__restrict__ cal,
                                   const double *__restrict__ freqs,
@@ -100,19 +110,32 @@ __global__ void bf_weights_kernel(bfk_dims d, const float *__restrict__ antpos,
         }
       } else if (d.prec == 2) {
         for (int s2 = 0; s2 < 2; s2++) {
-          uint32_t hi[8] = {0, 0, 0, 0, 0, 0, 0, 0}, lo[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          double x[32], r[32], lm = 0.0, rm = 0.0;
+          uint32_t ch[32], hi[8] = {0, 0, 0, 0, 0, 0, 0, 0}, lo[8] = {0, 0, 0, 0, 0, 0, 0, 0};
           for (int j = 0; j < 32; j++) {    // slot j: Wr (j < 16) or Wi of antenna j % 16
-            const double w = j < 16 ? wr[16 * s2 + j] : wi[16 * s2 + j - 16];
-            const uint32_t ch = e2m3_encode(w);
-            const uint32_t cl = e2m3_encode((w - e2m3_decode(ch)) * 16.0);
+            x[j] = j < 16 ? wr[16 * s2 + j] : wi[16 * s2 + j - 16];
+            lm = fmax(lm, fabs(x[j]));
+          }
+          const int kh = lane_boost(lm);
+          for (int j = 0; j < 32; j++) {
+            const double xb = ldexp(x[j], kh);
+            ch[j] = e2m3_encode(xb);
+            r[j] = xb - e2m3_decode(ch[j]);
+            rm = fmax(rm, fabs(r[j]));
+          }
+          const int kl = lane_boost(rm);
+          for (int j = 0; j < 32; j++) {
+            const uint32_t cl = e2m3_encode(ldexp(r[j], kl));
             const int bit = 6 * j, dw = bit >> 5, sh = bit & 31;
-            hi[dw] |= ch << sh;
+            hi[dw] |= ch[j] << sh;
             lo[dw] |= cl << sh;
             if (sh > 26) {
-              hi[dw + 1] |= ch >> (32 - sh);
+              hi[dw + 1] |= ch[j] >> (32 - sh);
               lo[dw + 1] |= cl >> (32 - sh);
             }
           }
+          hi[6] = (uint32_t)(127 - kh);        // E8M0 scales 2^-kh and 2^-(kh+kl)
+          hi[7] = (uint32_t)(127 - kh - kl);
           long rec = ((((long)(cq * d.nbt + bt) * d.nkc + kc) * 2 + s2) * 2) * 64 + lane;
           uint32_t *ph = (uint32_t *)(wfrag + rec * 32);
           uint32_t *pl = (uint32_t *)(wfrag + (rec + 64) * 32);
