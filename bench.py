@@ -33,10 +33,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "dsa110-xengine_amd", "python"), os.path.join(ROOT, "oracle")]
 
 METRIC = "beamformed input samples/s + % MFMA peak, FP8-split vs FP16, at +-1 LSB"
-PEAK_TFLOPS = {"fp16": 2500.0, "fp8split": 5000.0, "fp6split": 10100.0}   # MI355X dense MFMA
+PRECISIONS = ("fp16", "fp8split", "fp6split", "fp6split3")
+PEAK_TFLOPS = {"fp16": 2500.0, "fp8split": 5000.0, "fp6split": 10100.0,
+               "fp6split3": 10100.0}   # MI355X dense MFMA
+WEIGHT_PARTS = {"fp16": 1, "fp8split": 2, "fp6split": 2, "fp6split3": 3}   # MFMA passes
 DTYPES = {"fp16": "fp16 x fp16, fp32 accumulate",
           "fp8split": "fp8-e4m3 voltages x hi+lo fp8-e4m3 weights, fp32 accumulate",
-          "fp6split": "fp6-e2m3 voltages x hi+lo fp6-e2m3 weights, fp32 accumulate"}
+          "fp6split": "fp6-e2m3 voltages x hi+lo fp6-e2m3 weights, fp32 accumulate",
+          "fp6split3": "fp6-e2m3 voltages x hi+mid+lo fp6-e2m3 weights, fp32 accumulate"}
 NANT, NCHAN, NPKT, NBEAM, TAVG = 64, 384, 2048, 256, 32
 NSAMP = 2 * NPKT
 SAMPLES_PER_BLOCK = NANT * 2 * NCHAN * NSAMP
@@ -96,7 +100,7 @@ def workload(dsaxbf, nblocks, seed=2024):
 def time_path(dsaxbf, ev, prec_name, cfg, antpos, cal, theta, freqs, blocks, steps, warmup,
               variant="auto"):
     cfg.precision = {"fp16": dsaxbf.PREC_FP16, "fp8split": dsaxbf.PREC_FP8SPLIT,
-                     "fp6split": dsaxbf.PREC_FP6SPLIT}[prec_name]
+                     "fp6split": dsaxbf.PREC_FP6SPLIT, "fp6split3": dsaxbf.PREC_FP6SPLIT3}[prec_name]
     plan = dsaxbf.Plan(cfg)
     plan.set_weights(antpos, cal, freqs, theta)
     tune_ms = None
@@ -147,12 +151,13 @@ def time_path(dsaxbf, ev, prec_name, cfg, antpos, cal, theta, freqs, blocks, ste
     plan.execute(d_in[0], d_scale, d_out, d_pow, None, stream)
     out = d_out.download(np.empty((cfg.nout, NBEAM, cfg.ngroup), np.uint8), stream)
     pw = d_pow.download(np.empty((cfg.nout, NBEAM, cfg.ngroup), np.float32), stream)
+    wdev = plan.get_weights()            # the weights the kernel applied, for the error report
     plan.close()
     for b in d_in + [d_scale, d_out, d_pow, d_bp]:
         b.free()
     return dict(ms_per_step=total_ms / steps, kernel_ms=float(np.mean(kern_ms)),
                 kernel_ms_min=float(np.min(kern_ms)), out=out, power=pw, scale=scale,
-                variant=var_name,
+                variant=var_name, wdev=wdev,
                 autotune_ms=None if tune_ms is None else
                 {dsaxbf.VARIANT_NAMES[i]: float(t) for i, t in enumerate(tune_ms) if t > 0})
 
@@ -166,21 +171,33 @@ def errors(orc, cfg, block, W, res, groups):
     ref_u8 = orc.requant_block(ref, res["scale"])[:, :, sl].astype(np.int64)
     du = np.abs(res["out"][:, :, sl].astype(np.int64) - ref_u8)
     du = np.minimum(du, 256 - du)          # distance modulo the uint8 wrap
+    # the kernel alone: the oracle fed the weights the device applied (two groups)
+    gk = (groups[0], min(groups[0] + 2, groups[1]))
+    refk = orc.power(d, 1, block, res["wdev"], 0, *gk)[:, :, gk[0]:gk[1]]
+    relk = np.abs(res["power"][:, :, gk[0]:gk[1]] - refk) / np.abs(refk)
     return dict(max_rel_power_err=float(rel.max()), rms_rel_power_err=float(np.sqrt((rel ** 2).mean())),
+                meets_max_rel_1e3=bool(rel.max() <= 1e-3),
                 frac_u8_diff_ge1=float((du >= 1).mean()), frac_u8_diff_gt1=float((du > 1).mean()),
-                groups_checked=f"{groups[0]}..{groups[1] - 1}", mean_u8=float(res["out"].mean()))
+                meets_lsb_gt1_0p1pct=bool((du > 1).mean() <= 1e-3),
+                groups_checked=f"{groups[0]}..{groups[1] - 1}", mean_u8=float(res["out"].mean()),
+                kernel_vs_own_weights_max_rel=float(relk.max()),
+                kernel_groups_checked=f"{gk[0]}..{gk[1] - 1}")
 
 
 def cpu_baseline(orc, cfg, block, W, scale, threads, target_s=10.0):
+    import tempfile
     d = orc.dims(NANT, NCHAN, NPKT, NBEAM, TAVG)
+    tmp = tempfile.mkdtemp(prefix="dsaxbf_cpu_")
+    native = orc.native_cpu_lib(tmp)        # -march=native for this host's cores
     secs, reps = 0.0, 0
     while secs < target_s and reps < 40:
-        s, _, _ = orc.cpu_beamform(d, 1, block, W, scale, 0, cfg.ngroup, threads)
+        s, _, _ = orc.cpu_beamform(d, 1, block, W, scale, 0, cfg.ngroup, threads, native)
         secs += s
         reps += 1
+    build = "-O3 -march=native" if native is not None else "-O3 -march=x86-64-v4"
     return dict(value=reps * SAMPLES_PER_BLOCK / secs, unit="samples/s", cores=threads, kind="port",
                 sample=f"{reps} full block(s) of the same workload, FP32 pthreads C loop "
-                       f"(oracle/bf_cpu.c), {secs:.1f} s total, {threads} threads on "
+                       f"(oracle/bf_cpu.c, {build}), {secs:.1f} s total, {threads} threads on "
                        f"{cpu_model()} ({os.cpu_count()} logical CPUs visible)",
                 gflops=reps * FLOP_PER_BLOCK / secs / 1e9, s_per_block=secs / reps)
 
@@ -320,7 +337,7 @@ def batch_bench(dsaxbf, ev, seed=2025, nblocks=24, reps=5):
     flop = sum(8.0 * b * 64 * c * 2 * t for c, b, t in shapes)
     samples = sum(64 * 2 * c * t for c, b, t in shapes)
     for prec_name, prec in (("fp16", dsaxbf.PREC_FP16), ("fp8split", dsaxbf.PREC_FP8SPLIT),
-                            ("fp6split", dsaxbf.PREC_FP6SPLIT)):
+                            ("fp6split", dsaxbf.PREC_FP6SPLIT), ("fp6split3", dsaxbf.PREC_FP6SPLIT3)):
         plans, bufs = [], []
         for i, (c, b, t) in enumerate(shapes):
             cfg = dsaxbf.config(nant=64, nchan=c, npkt=t // 2, nbeam=b, tavg=32, precision=prec,
@@ -386,12 +403,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--precision", choices=["fp8split", "fp16", "fp6split"], default="fp8split")
+    ap.add_argument("--precision", choices=list(PRECISIONS), default="fp8split")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the oracle error report")
     ap.add_argument("--check-groups", type=int, default=8)
     ap.add_argument("--variant", default="auto",
-                    choices=["auto", "classic256", "pipelined256", "classic128s", "classic128",
+                    choices=["auto", "classic256", "classic128s", "classic128",
                              "classic64", "shared256"],
                     help="fused-kernel variant (auto: the library's autotuner picks)")
     ap.add_argument("--no-batch", action="store_true",
@@ -423,7 +440,7 @@ def main():
     cfg, antpos, cal, theta, freqs, blocks, tone_beam = workload(dsaxbf, 3)
     print(f"# workload ready in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     res = {}
-    for prec in ("fp16", "fp8split", "fp6split"):
+    for prec in PRECISIONS:
         res[prec] = time_path(dsaxbf, ev, prec, cfg, antpos, cal, theta, freqs, blocks,
                               args.steps, args.warmup, args.variant)
         print(f"# {prec}: {res[prec]['ms_per_step']:.4f} ms/step, kernel "
@@ -471,8 +488,7 @@ def main():
     r = res[main_p]
     value = SAMPLES_PER_BLOCK / (r["ms_per_step"] / 1e3)
     achieved = FLOP_PER_BLOCK / (r["kernel_ms"] / 1e3) / 1e12
-    kernel = {"pipelined256": "bf_pipelined_kernel",
-              "shared256": "bf_shared_kernel"}.get(r["variant"], "bf_classic_kernel")
+    kernel = {"shared256": "bf_shared_kernel"}.get(r["variant"], "bf_classic_kernel")
     line = {
         "metric": METRIC, "value": value, "unit": "samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
@@ -499,12 +515,13 @@ def main():
                       "ms_per_step": res[p]["ms_per_step"], "kernel_ms": res[p]["kernel_ms"],
                       "mfma_frac_of_own_peak": FLOP_PER_BLOCK / (res[p]["kernel_ms"] / 1e3) / 1e12
                       / PEAK_TFLOPS[p],
-                      "executed_mfma_frac": (1.0 if p == "fp16" else 2.0) * FLOP_PER_BLOCK
+                      "executed_mfma_frac": WEIGHT_PARTS[p] * FLOP_PER_BLOCK
                       / (res[p]["kernel_ms"] / 1e3) / 1e12 / PEAK_TFLOPS[p],
                       "variant": res[p]["variant"], "autotune_ms": res[p]["autotune_ms"],
                       "errors_vs_fp64_oracle": errs.get(p)} for p in res},
         "fp8split_speedup_vs_fp16": res["fp16"]["ms_per_step"] / res["fp8split"]["ms_per_step"],
         "fp6split_speedup_vs_fp16": res["fp16"]["ms_per_step"] / res["fp6split"]["ms_per_step"],
+        "fp6split3_speedup_vs_fp16": res["fp16"]["ms_per_step"] / res["fp6split3"]["ms_per_step"],
         "tone_check": {k: errs[k] for k in ("tone_beam", "peak_beam_fp16") if k in errs},
         "scaled_array_stream": stream,
         "uneven_batch_graph": batch,

@@ -72,30 +72,32 @@ __global__ __launch_bounds__(256) void bf_stuff_kernel(bfk_dims d, const uint8_t
 
 // beam_power[b] = sum over 32-sample units and groups of bpart[b][.][.]
 // (badder + calc_allbp, dsaX_beamformer.cu:78-112,482-495).  One workgroup
-// per beam: each thread's loads are all issued before its FP64 sum, the waves
-// reduce with cross-lane swaps and the four wave sums are added in LDS, all
-// in a fixed order, so the result is deterministic.
+// per beam, FP32 throughout: each thread sums its strided share in eight
+// independent partial sums (all loads of a round issued before the adds),
+// the waves reduce with cross-lane swaps and the four wave sums are added in
+// LDS, all in a fixed order, so the result is deterministic.
 __global__ __launch_bounds__(256) void bf_beam_power_kernel(bfk_dims d, const float *__restrict__ bpart,
                                                             float *__restrict__ bp) {
-  __shared__ double red[4];
+  __shared__ float red[4];
   const int b = blockIdx.x, tid = threadIdx.x;
   const int n = (d.nsamp / 32) * d.ngroup;
   const float *src = bpart + (long)b * n;
-  double s = 0.0;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   int i = tid;
   for (; i + 7 * 256 < n; i += 8 * 256) {
     float v[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) v[k] = src[i + k * 256];
 #pragma unroll
-    for (int k = 0; k < 8; k++) s += v[k];
+    for (int k = 0; k < 8; k++) acc[k] += v[k];
   }
-  for (; i < n; i += 256) s += src[i];
+  for (; i < n; i += 256) acc[0] += src[i];
+  float s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m);
   if ((tid & 63) == 0) red[tid >> 6] = s;
   __syncthreads();
-  if (tid == 0) bp[b] = (float)((red[0] + red[1]) + (red[2] + red[3]));
+  if (tid == 0) bp[b] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 extern "C" int bfk_stuff(const bfk_dims *d, const uint8_t *d_in, const float *d_scale,

@@ -35,7 +35,12 @@ int bfk_check(hipError_t e, const char *what);
 //   16..31 = Wi; hi and lo records at the FP8 indices, the operand in bytes
 //   0..23; the hi record's dwords 6 and 7 hold the lane's E8M0 B-operand
 //   scales of the hi and lo parts.
-// All formats use 128 bytes per lane per (cq, beam tile, antenna chunk).
+// FP6 three-part (hi + mid + lo E2M3 weights): hi, mid and lo records per
+//   step s2 at index ((((cq*nbt + bt)*nkc + kc)*2 + s2)*3 + part)*64 + lane;
+//   the hi record's dwords 6 and 7 hold the hi and mid E8M0 scales, the mid
+//   record's dword 6 the lo scale.
+// All formats but the three-part one use 128 bytes per lane per (cq, beam
+// tile, antenna chunk); it uses 192.
 // The matching A (voltage) slots are built by unpack_f16 / nib_e4m3 / the
 // FP6 converter (unpack_cq_f6).
 // ---------------------------------------------------------------------------
@@ -391,9 +396,44 @@ template <int NB, int NK> struct WFrag<2, NB, NK> {
   }
 };
 
-// One 32x32x64 MFMA of the FP8 (PREC 1) or FP6 (PREC 2) chains.  FP6 weight
-// parts are stored 2^k larger per lane; the lane's E8M0 B scale sb = 127 - k
-// takes that back (A scale 2^0).  FP8 uses no scales.
+// FP6 hi + mid + lo fragments (three-part split): the hi fragment's dwords 6
+// and 7 carry the E8M0 scales of the hi and mid parts, the mid fragment's
+// dword 6 that of the lo part.
+template <int NB, int NK> struct WFrag<3, NB, NK> {
+  i8v hi[NB][NK][2], mid[NB][NK][2], lo[NB][NK][2];
+  float sc[NB];
+  __device__ __forceinline__ int shi(int bt, int kc, int s2) const { return hi[bt][kc][s2][6]; }
+  __device__ __forceinline__ int smid(int bt, int kc, int s2) const { return hi[bt][kc][s2][7]; }
+  __device__ __forceinline__ int slo(int bt, int kc, int s2) const { return mid[bt][kc][s2][6]; }
+  __device__ __forceinline__ void load_k(const uint8_t *wfrag, long wrow_bt, int bt, int kc, int s2, int lane) {
+    const long rec = ((((wrow_bt + bt) * NK + kc) * 2 + s2) * 3) * 64 + lane;
+    hi[bt][kc][s2] = *(const i8v *)(wfrag + rec * 32);
+    mid[bt][kc][s2] = *(const i8v *)(wfrag + (rec + 64) * 32);
+    lo[bt][kc][s2] = WFrag<2, NB, NK>::ld6(wfrag + (rec + 128) * 32);
+  }
+  __device__ __forceinline__ void load_scale(const float *wscale, long sc_off, int n) {
+#pragma unroll
+    for (int bt = 0; bt < NB; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
+  }
+  __device__ __forceinline__ void load(const uint8_t *wfrag, const float *wscale, long wrow_bt,
+                                       long sc_off, int lane, int n) {
+#pragma unroll
+    for (int bt = 0; bt < NB; bt++)
+#pragma unroll
+      for (int kc = 0; kc < NK; kc++)
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) load_k(wfrag, wrow_bt, bt, kc, s2, lane);
+    load_scale(wscale, sc_off, n);
+  }
+  __device__ __forceinline__ void load_one(const uint8_t *wfrag, long wrow_bt, int bt, int s2, int lane) {
+    static_assert(NK == 1, "single-chunk tiles");
+    load_k(wfrag, wrow_bt, bt, 0, s2, lane);
+  }
+};
+
+// One 32x32x64 MFMA of the FP8 (PREC 1) or FP6 (PREC 2, 3) chains.  FP6
+// weight parts are stored 2^k larger per lane; the lane's E8M0 B scale
+// sb = 127 - k takes that back (A scale 2^0).  FP8 uses no scales.
 template <int PREC>
 __device__ __forceinline__ f16v mfma64(const i8v &a, const i8v &b, const f16v &c, int sb) {
   if constexpr (PREC == 1) return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, 0, 0, 0);

@@ -74,11 +74,6 @@ extern "C" int bfk_batch_prepare(const bfk_dims *dims, const bfk_batch_bufs *buf
       bfk_set_error("bfk_batch_prepare: block %d or its weights above 2 GiB", i);
       return -1;
     }
-    if (d->variant == BFK_VAR_PIPELINED_256) {
-      bfk_set_error("bfk_batch_prepare: block %d uses the pipelined variant; batches run classic or "
-                    "shared tiles", i);
-      return -1;
-    }
     if (!bfk_variant_supported(d, d->variant)) {
       bfk_set_error("bfk_batch_prepare: block %d: variant %d cannot run its shape", i, d->variant);
       return -1;

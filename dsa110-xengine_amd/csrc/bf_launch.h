@@ -28,7 +28,7 @@ typedef struct bfk_dims {
   int ncal;       /* weight channels: ngroup (group mode) or nchan (fine) */
   int wmode;      /* 0 group, 1 fine                                      */
   int amode;      /* 0 theta, 1 sin(theta)                                */
-  int prec;       /* 0 FP16, 1 FP8 hi+lo, 2 FP6 hi+lo                     */
+  int prec;       /* 0 FP16, 1 FP8 hi+lo, 2 FP6 hi+lo, 3 FP6 hi+mid+lo    */
   int order;      /* 0 capture, 1 split -r                                */
   int stuff;      /* 0 none, 1 antennas, 2 incoherent beam 0              */
   int variant;    /* fused-kernel variant, BFK_VAR_*                      */
@@ -39,13 +39,12 @@ typedef struct bfk_dims {
  *              beams per tile (128S: 2 beam tiles per wave, wave pairs split a
  *              tile's samples; 128: one beam tile per wave; 64: one beam tile
  *              per wave, split);
- * PIPELINED_256 one persistent workgroup per CU, staging and weight loads
- *              overlapped with the MFMA work in software (nant <= 64);
- * SHARED_256   8 waves per tile that unpack each (channel, pol) once into
- *              LDS operands all waves read (nant <= 64). */
+ * SHARED_256   waves of a tile that unpack each unit once into LDS
+ *              operands all of them read (4 waves to 64 antennas, 8 above).
+ * Value 1 (a persistent software-pipelined schedule, never faster) is
+ * retired: no shape supports it. */
 enum {
   BFK_VAR_CLASSIC_256 = 0,
-  BFK_VAR_PIPELINED_256 = 1,
   BFK_VAR_CLASSIC_128S = 2,
   BFK_VAR_CLASSIC_128 = 3,
   BFK_VAR_CLASSIC_64 = 4,

@@ -184,17 +184,20 @@ __device__ __forceinline__ void unit16(Acc<BTW> &A, WFrag<0, BTW, 1> &W, const u
   __builtin_amdgcn_s_setprio(0);
 }
 
-// FP8 chain: beam tile BT of one 32-sample unit, the unit's A_re / A_im for
-// both K steps given, accumulated over K in the order of gemm_unit (hi then lo
-// weight part per step), so results are bit-identical to the classic
-// schedule.  Between its MFMAs it detects the pending chain into `into`, two
-// elements per MFMA.  With PF, each K step of W[BT] is reloaded from row wnext
+// FP8 / FP6 chain: beam tile BT of one 32-sample unit, the unit's A_re /
+// A_im for both K steps given, accumulated over K in the order of gemm_unit
+// (per step the weight parts hi, then lo; or hi, mid, lo for the three-part
+// FP6 split), so FP8 results are bit-identical to the classic schedule.
+// Between its MFMAs it detects the pending chain into `into`, two elements
+// after each MFMA from the second on, in element order (so the sums are those
+// of detect1).  With PF, each K step of W[BT] is reloaded from row wnext
 // right after its last MFMA (the rolling prefetch of unit16).  Scheduling
 // barriers keep this interleave as written.
 template <int PREC, int BT, bool PF, int NP>
 __device__ __forceinline__ void chain8(f16v &cr, f16v &ci, WFrag<PREC, BTW, 1> &W, const i8v (&are)[2],
                                        const i8v (&aim)[2], int lane, const uint8_t *wfrag, long wnext,
                                        const Pending &pd, float (&into)[NP]) {
+  constexpr int NPART = PREC == 3 ? 3 : 2;
   float sd[NP][4];
 #pragma unroll
   for (int j = 0; j < NP; j++) sd[j][0] = sd[j][1] = sd[j][2] = sd[j][3] = 0.f;
@@ -202,26 +205,30 @@ __device__ __forceinline__ void chain8(f16v &cr, f16v &ci, WFrag<PREC, BTW, 1> &
   ci = (f16v){};
 #pragma unroll
   for (int s2 = 0; s2 < 2; s2++) {
-    const int k = 4 * s2;   // MFMA slot of this step's first MFMA
-    cr = mfma64<PREC>(are[s2], W.hi[BT][0][s2], cr, W.shi(BT, 0, s2));
-    if (k > 0) { det_elem<NP>(pd, 2 * k - 2, sd); det_elem<NP>(pd, 2 * k - 1, sd); }
-    __builtin_amdgcn_sched_barrier(0);
-    ci = mfma64<PREC>(aim[s2], W.hi[BT][0][s2], ci, W.shi(BT, 0, s2));
-    det_elem<NP>(pd, 2 * k, sd);
-    det_elem<NP>(pd, 2 * k + 1, sd);
-    __builtin_amdgcn_sched_barrier(0);
-    cr = mfma64<PREC>(are[s2], W.lo[BT][0][s2], cr, W.slo(BT, 0, s2));
-    det_elem<NP>(pd, 2 * k + 2, sd);
-    det_elem<NP>(pd, 2 * k + 3, sd);
-    __builtin_amdgcn_sched_barrier(0);
-    ci = mfma64<PREC>(aim[s2], W.lo[BT][0][s2], ci, W.slo(BT, 0, s2));
-    det_elem<NP>(pd, 2 * k + 4, sd);
-    det_elem<NP>(pd, 2 * k + 5, sd);
-    if constexpr (PF) W.load_one(wfrag, wnext, BT, s2, lane);
-    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int pt = 0; pt < NPART; pt++) {
+      const int m = (s2 * NPART + pt) * 2;   // index of this pair's first MFMA in the chain
+      const i8v *w;
+      int sb;
+      if constexpr (PREC == 3) {
+        w = pt == 0 ? &W.hi[BT][0][s2] : pt == 1 ? &W.mid[BT][0][s2] : &W.lo[BT][0][s2];
+        sb = pt == 0 ? W.shi(BT, 0, s2) : pt == 1 ? W.smid(BT, 0, s2) : W.slo(BT, 0, s2);
+      } else {
+        w = pt == 0 ? &W.hi[BT][0][s2] : &W.lo[BT][0][s2];
+        sb = pt == 0 ? W.shi(BT, 0, s2) : W.slo(BT, 0, s2);
+      }
+      cr = mfma64<PREC>(are[s2], *w, cr, sb);
+      if (m > 0 && 2 * m - 1 < 16) { det_elem<NP>(pd, 2 * m - 2, sd); det_elem<NP>(pd, 2 * m - 1, sd); }
+      __builtin_amdgcn_sched_barrier(0);
+      ci = mfma64<PREC>(aim[s2], *w, ci, sb);
+      if (2 * m + 1 < 16) { det_elem<NP>(pd, 2 * m, sd); det_elem<NP>(pd, 2 * m + 1, sd); }
+      if (PF && pt == NPART - 1) W.load_one(wfrag, wnext, BT, s2, lane);
+      __builtin_amdgcn_sched_barrier(0);
+    }
   }
-  det_elem<NP>(pd, 14, sd);
-  det_elem<NP>(pd, 15, sd);
+  constexpr int DONE = 4 * NPART * 2 - 2;   // elements detected between the MFMAs
+#pragma unroll
+  for (int v = DONE; v < 16; v++) det_elem<NP>(pd, v, sd);
   det_final<NP>(sd, pd.sc, into);
 }
 
@@ -441,7 +448,8 @@ extern "C" int bfk_beamform_shared(const bfk_dims *d, const uint8_t *d_in, const
   const dim3 grid((a.ntiles + 7) / 8 * 8);
   if (d->prec == 0) launch<0>(d->tavg, grid, (hipStream_t)stream, a);
   else if (d->prec == 1) launch<1>(d->tavg, grid, (hipStream_t)stream, a);
-  else launch<2>(d->tavg, grid, (hipStream_t)stream, a);
+  else if (d->prec == 2) launch<2>(d->tavg, grid, (hipStream_t)stream, a);
+  else launch<3>(d->tavg, grid, (hipStream_t)stream, a);
   return bfk_check(hipGetLastError(), "bf_shared_kernel");
 }
 
@@ -458,6 +466,7 @@ extern "C" int bfk_beamform_shared_batch(int prec, int tavg, const void *d_jobs,
   const dim3 grid((ntiles + 7) / 8 * 8);
   if (prec == 0) launch_batch<0>(tavg, grid, (hipStream_t)stream, (const BfArgs *)d_jobs, d_tile_start, njobs);
   else if (prec == 1) launch_batch<1>(tavg, grid, (hipStream_t)stream, (const BfArgs *)d_jobs, d_tile_start, njobs);
-  else launch_batch<2>(tavg, grid, (hipStream_t)stream, (const BfArgs *)d_jobs, d_tile_start, njobs);
+  else if (prec == 2) launch_batch<2>(tavg, grid, (hipStream_t)stream, (const BfArgs *)d_jobs, d_tile_start, njobs);
+  else launch_batch<3>(tavg, grid, (hipStream_t)stream, (const BfArgs *)d_jobs, d_tile_start, njobs);
   return bfk_check(hipGetLastError(), "bf_shared_batch_kernel");
 }

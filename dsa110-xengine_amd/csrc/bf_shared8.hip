@@ -219,6 +219,10 @@ __device__ __forceinline__ void unit8w(Acc<1> &A, WFrag<PREC, 1, NKC> &W, const 
         }
         A.r[0] = mfma64<PREC>(are, W.hi[0][kc][s2], A.r[0], W.shi(0, kc, s2));
         A.i[0] = mfma64<PREC>(aim, W.hi[0][kc][s2], A.i[0], W.shi(0, kc, s2));
+        if constexpr (PREC == 3) {
+          A.r[0] = mfma64<PREC>(are, W.mid[0][kc][s2], A.r[0], W.smid(0, kc, s2));
+          A.i[0] = mfma64<PREC>(aim, W.mid[0][kc][s2], A.i[0], W.smid(0, kc, s2));
+        }
         A.r[0] = mfma64<PREC>(are, W.lo[0][kc][s2], A.r[0], W.slo(0, kc, s2));
         A.i[0] = mfma64<PREC>(aim, W.lo[0][kc][s2], A.i[0], W.slo(0, kc, s2));
         if constexpr (PF) {
@@ -248,7 +252,7 @@ __device__ __forceinline__ void shared8_tile(const BfArgs &a, int L, uint8_t *st
   // Next sub-step's voltages held in registers during the current one: only
   // where that fits beside the weights (128 antennas, FP8 and FP6); elsewhere
   // each sub-step loads its own after the previous one.
-  constexpr bool PFS = PREC >= 1 && NKC == 2;
+  constexpr bool PFS = (PREC == 1 || PREC == 2) && NKC == 2;
 
   // sub-step 0's voltages and the first weights; each later sub-step's
   // voltages are loaded while the previous one is computed
@@ -373,6 +377,12 @@ static int shared8_shape_ok(const bfk_dims *d) {
     bfk_set_error("bfk_beamform: 8-wave shared schedule needs 65-256 antennas (nant_pad=%d)", d->nant_pad);
     return 0;
   }
+  if (d->prec == 3 && d->nkc != 2) {
+    // three weight parts of four antenna chunks would take 192 registers
+    bfk_set_error("bfk_beamform: the three-part FP6 split runs up to 128 antennas (nant_pad=%d)",
+                  d->nant_pad);
+    return 0;
+  }
   return 1;
 }
 
@@ -386,7 +396,8 @@ extern "C" int bfk_beamform_shared8(const bfk_dims *d, const uint8_t *d_in, cons
   if (d->nkc == 2) {
     if (d->prec == 0) launch8<0, 2>(d->tavg, grid, s, a);
     else if (d->prec == 1) launch8<1, 2>(d->tavg, grid, s, a);
-    else launch8<2, 2>(d->tavg, grid, s, a);
+    else if (d->prec == 2) launch8<2, 2>(d->tavg, grid, s, a);
+    else launch8<3, 2>(d->tavg, grid, s, a);
   } else {
     if (d->prec == 0) launch8<0, 4>(d->tavg, grid, s, a);
     else if (d->prec == 1) launch8<1, 4>(d->tavg, grid, s, a);
@@ -411,7 +422,8 @@ extern "C" int bfk_beamform_shared8_batch(int prec, int tavg, int nkc, const voi
   if (nkc == 2) {
     if (prec == 0) launch8_batch<0, 2>(tavg, grid, s, jobs, d_tile_start, njobs);
     else if (prec == 1) launch8_batch<1, 2>(tavg, grid, s, jobs, d_tile_start, njobs);
-    else launch8_batch<2, 2>(tavg, grid, s, jobs, d_tile_start, njobs);
+    else if (prec == 2) launch8_batch<2, 2>(tavg, grid, s, jobs, d_tile_start, njobs);
+    else launch8_batch<3, 2>(tavg, grid, s, jobs, d_tile_start, njobs);
   } else {
     if (prec == 0) launch8_batch<0, 4>(tavg, grid, s, jobs, d_tile_start, njobs);
     else if (prec == 1) launch8_batch<1, 4>(tavg, grid, s, jobs, d_tile_start, njobs);

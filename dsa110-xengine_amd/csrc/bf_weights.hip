@@ -1,7 +1,8 @@
 // bf_weights.hip -- beam weights on the device (replaces calc_weights,
 // dsaX_beamformer.cu:420-449): FP64 phase exp(-2 pi i f ang x / c) times the
 // calibration, scaled by a power of two per (channel, pol, beam) and written
-// as FP16, FP8 E4M3 hi+lo or FP6 E2M3 hi+lo MFMA B-operand fragments.
+// as FP16, FP8 E4M3 hi+lo or FP6 E2M3 hi+lo (or hi+mid+lo) MFMA B-operand
+// fragments.
 #include <stdarg.h>
 
 #include "bf_common.h"
@@ -21,18 +22,19 @@ int bfk_check(hipError_t e, const char *what) {
 }
 
 extern "C" size_t bfk_wfrag_bytes(const bfk_dims *d) {
-  return (size_t)d->ncal * 2 * d->nbt * d->nkc * 64 * WFRAG_LANE_BYTES;
+  return (size_t)d->ncal * 2 * d->nbt * d->nkc * 64 * (d->prec == 3 ? 3 * WFRAG_LANE_BYTES / 2 : WFRAG_LANE_BYTES);
 }
 extern "C" size_t bfk_wscale_bytes(const bfk_dims *d) {
   return (size_t)d->ncal * 2 * d->nbeam_pad * sizeof(float);
 }
 
-// The largest k in 0..60 with m * 2^k <= 7.5 (0 for m = 0).
-__device__ static inline int lane_boost(double m) {
+// The largest k in 0..kmax with m * 2^k <= 7.5 (0 for m = 0).
+__device__ static inline int lane_boost(double m, int kmax) {
   if (!(m > 0.0)) return 0;
   int k = (int)floor(log2(7.5 / m));
-  k = k < 0 ? 0 : k > 60 ? 60 : k;
+  k = k < 0 ? 0 : k > kmax ? kmax : k;
   while (k > 0 && ldexp(m, k) > 7.5) k--;
+  while (k < kmax && ldexp(m, k + 1) <= 7.5) k++;
   return k;
 }
 
@@ -42,8 +44,9 @@ __device__ static inline int lane_boost(double m) {
 // that undoes it and the voltage code's unit (2^-9 for FP8, 1/8 for FP6).
 // E2M3 has only two exponent bits, so FP6 also boosts each lane's 32 slots
 // (16 antennas of one beam and K step) by the largest power of two that keeps
-// them within 7.5, and its lo part by the largest that keeps the lane's
-// residuals within 7.5; the lane's E8M0 MFMA operand scales take both back.
+// them within 7.5, and each further part (lo; or mid, then lo) by the
+// largest that keeps the lane's remaining residuals within 7.5; the lane's
+// E8M0 MFMA operand scales take them back (their exponents stay >= 0).
 __global__ void bf_weights_kernel(bfk_dims d, const float *__restrict__ antpos,
                                   const float *__restrict__ cal,
                                   const double *__restrict__ freqs,
@@ -76,7 +79,7 @@ __global__ void bf_weights_kernel(bfk_dims d, const float *__restrict__ antpos,
   if (mx > 0.0) frexp(mx, &k);
   int e = (d.prec == 0 ? 15 : 8) - k;
   double sc = ldexp(1.0, e);
-  if (d.prec == 2) sc = mx > 0.0 ? 7.5 / mx : 1.0;
+  if (d.prec >= 2) sc = mx > 0.0 ? 7.5 / mx : 1.0;
   wscale[idx] = (float)(d.prec == 0   ? ldexp(1.0, -2 * e)
                         : d.prec == 1 ? ldexp(1.0, 18 - 2 * e)
                                       : 64.0 / (sc * sc));
@@ -108,38 +111,38 @@ __global__ void bf_weights_kernel(bfk_dims d, const float *__restrict__ antpos,
           long rec = (((long)(cq * d.nbt + bt) * d.nkc + kc) * 8 + s) * 64 + lane;
           *(h8v *)(wfrag + rec * 16) = v;
         }
-      } else if (d.prec == 2) {
+      } else if (d.prec >= 2) {
+        const int nparts = d.prec == 3 ? 3 : 2;
         for (int s2 = 0; s2 < 2; s2++) {
-          double x[32], r[32], lm = 0.0, rm = 0.0;
-          uint32_t ch[32], hi[8] = {0, 0, 0, 0, 0, 0, 0, 0}, lo[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-          for (int j = 0; j < 32; j++) {    // slot j: Wr (j < 16) or Wi of antenna j % 16
-            x[j] = j < 16 ? wr[16 * s2 + j] : wi[16 * s2 + j - 16];
-            lm = fmax(lm, fabs(x[j]));
-          }
-          const int kh = lane_boost(lm);
-          for (int j = 0; j < 32; j++) {
-            const double xb = ldexp(x[j], kh);
-            ch[j] = e2m3_encode(xb);
-            r[j] = xb - e2m3_decode(ch[j]);
-            rm = fmax(rm, fabs(r[j]));
-          }
-          const int kl = lane_boost(rm);
-          for (int j = 0; j < 32; j++) {
-            const uint32_t cl = e2m3_encode(ldexp(r[j], kl));
-            const int bit = 6 * j, dw = bit >> 5, sh = bit & 31;
-            hi[dw] |= ch[j] << sh;
-            lo[dw] |= cl << sh;
-            if (sh > 26) {
-              hi[dw + 1] |= ch[j] >> (32 - sh);
-              lo[dw + 1] |= cl >> (32 - sh);
+          double r[32];
+          uint32_t code[3][8];
+          int ktot = 0, sexp[3];
+          for (int j = 0; j < 32; j++)      // slot j: Wr (j < 16) or Wi of antenna j % 16
+            r[j] = j < 16 ? wr[16 * s2 + j] : wi[16 * s2 + j - 16];
+          for (int pt = 0; pt < nparts; pt++) {
+            double m = 0.0;
+            for (int j = 0; j < 32; j++) m = fmax(m, fabs(r[j]));
+            const int k = lane_boost(m, 127 - ktot < 60 ? 127 - ktot : 60);
+            ktot += k;
+            sexp[pt] = 127 - ktot;          // E8M0 scale 2^-ktot of this part
+            for (int u = 0; u < 8; u++) code[pt][u] = 0;
+            for (int j = 0; j < 32; j++) {
+              const double rb = ldexp(r[j], k);
+              const uint32_t c = e2m3_encode(rb);
+              r[j] = rb - e2m3_decode(c);
+              const int bit = 6 * j, dw = bit >> 5, sh = bit & 31;
+              code[pt][dw] |= c << sh;
+              if (sh > 26) code[pt][dw + 1] |= c >> (32 - sh);
             }
           }
-          hi[6] = (uint32_t)(127 - kh);        // E8M0 scales 2^-kh and 2^-(kh+kl)
-          hi[7] = (uint32_t)(127 - kh - kl);
-          long rec = ((((long)(cq * d.nbt + bt) * d.nkc + kc) * 2 + s2) * 2) * 64 + lane;
-          uint32_t *ph = (uint32_t *)(wfrag + rec * 32);
-          uint32_t *pl = (uint32_t *)(wfrag + (rec + 64) * 32);
-          for (int u = 0; u < 8; u++) { ph[u] = hi[u]; pl[u] = lo[u]; }
+          code[0][6] = (uint32_t)sexp[0];
+          code[0][7] = (uint32_t)sexp[1];
+          if (nparts == 3) code[1][6] = (uint32_t)sexp[2];
+          long rec = ((((long)(cq * d.nbt + bt) * d.nkc + kc) * 2 + s2) * nparts) * 64 + lane;
+          for (int pt = 0; pt < nparts; pt++) {
+            uint32_t *pp = (uint32_t *)(wfrag + (rec + 64 * pt) * 32);
+            for (int u = 0; u < 8; u++) pp[u] = code[pt][u];
+          }
         }
       } else {
         for (int s2 = 0; s2 < 2; s2++) {

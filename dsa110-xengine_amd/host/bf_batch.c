@@ -50,8 +50,6 @@ static int build(dsaxbf_batch *b) {
     if (rc) return rc;
     if (b->dims[i].stuff != DSAXBF_STUFF_NONE)
       return dsaxbf_fail(DSAXBF_EINVAL, "block %d: -s/-q modes are not batched", i);
-    if (b->dims[i].variant == BFK_VAR_PIPELINED_256)
-      b->dims[i].variant = BFK_VAR_CLASSIC_256;   /* batches run classic or shared tiles */
   }
   if (bfk_batch_prepare(b->dims, b->bufs, b->n, &b->kern))
     return dsaxbf_fail(DSAXBF_EINVAL, "%s", bfk_last_error());
@@ -107,6 +105,11 @@ int dsaxbf_batch_create(dsaxbf_batch **out, dsaxbf_plan *const *plans, int n,
       return dsaxbf_fail(DSAXBF_EINVAL, "block %d: NULL plan or buffer", i);
     dsaxbf_get_config(plans[i], &c);
     if (c.device != c0.device) return dsaxbf_fail(DSAXBF_EINVAL, "block %d is on another device", i);
+    /* a plan owns the per-unit beam-power scratch its block writes, so two
+       blocks of one launch cannot share a plan */
+    for (int k = 0; k < i; k++)
+      if (plans[k] == plans[i])
+        return dsaxbf_fail(DSAXBF_EINVAL, "block %d reuses the plan of block %d", i, k);
   }
   if (hipSetDevice(c0.device) != hipSuccess) return dsaxbf_fail(DSAXBF_EHIP, "hipSetDevice");
   dsaxbf_batch *b = calloc(1, sizeof(*b));
@@ -182,7 +185,7 @@ int dsaxbf_batch_autotune(dsaxbf_batch *b, int reps, float *ms) {
       }
     if (best[i] >= 0) continue;
     rc = dsaxbf_time_variants(b->plans[i], b->bufs[i].d_in, b->bufs[i].d_scale, b->bufs[i].d_out, reps,
-                              1, ms ? ms + (size_t)i * DSAXBF_NVARIANT : NULL, &best[i]);
+                              ms ? ms + (size_t)i * DSAXBF_NVARIANT : NULL, &best[i]);
   }
   for (int i = 0; i < b->n && rc == DSAXBF_OK; i++) rc = dsaxbf_set_variant(b->plans[i], best[i]);
   free(best);

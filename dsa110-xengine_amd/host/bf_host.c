@@ -71,8 +71,11 @@ int dsaxbf_check_config(const dsaxbf_config *c) {
   if (c->tavg != 8 && c->tavg != 16 && c->tavg != 32 && c->tavg != 64)
     return dsaxbf_fail(DSAXBF_EINVAL, "tavg=%d must be 8, 16, 32 or 64", c->tavg);
   if (c->precision != DSAXBF_PREC_FP16 && c->precision != DSAXBF_PREC_FP8SPLIT &&
-      c->precision != DSAXBF_PREC_FP6SPLIT)
+      c->precision != DSAXBF_PREC_FP6SPLIT && c->precision != DSAXBF_PREC_FP6SPLIT3)
     return dsaxbf_fail(DSAXBF_EINVAL, "precision=%d unknown", c->precision);
+  if (c->precision == DSAXBF_PREC_FP6SPLIT3 && c->nant > 128)
+    return dsaxbf_fail(DSAXBF_EINVAL, "nant=%d: the three-part FP6 split runs up to 128 antennas",
+                       c->nant);
   if (c->weight_mode != 0 && c->weight_mode != 1)
     return dsaxbf_fail(DSAXBF_EINVAL, "weight_mode=%d unknown", c->weight_mode);
   if (c->angle_mode != 0 && c->angle_mode != 1)

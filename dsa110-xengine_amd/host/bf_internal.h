@@ -15,10 +15,9 @@ struct bfk_dims;
 int dsaxbf_plan_internals(dsaxbf_plan *p, struct bfk_dims *dims, void **d_wfrag, float **d_wscale,
                           float **d_bpart);
 /* Time each variant the plan's shape supports on the given device buffers
- * (reps launches after one warm-up; batchable skips the pipelined one, which
- * batches do not run);
+ * (reps launches after one warm-up);
  * ms[v] = mean ms or -1, *best = fastest.  The plan's variant is unchanged. */
 int dsaxbf_time_variants(dsaxbf_plan *p, const uint8_t *d_in, const float *d_scale, uint8_t *d_out,
-                         int reps, int batchable, float *ms, int *best);
+                         int reps, float *ms, int *best);
 
 #endif

@@ -61,7 +61,7 @@ static void fill_dims(const dsaxbf_config *c, bfk_dims *d) {
   /* the autotuner's usual picks on MI355X: shared 256-beam tiles from 129
    * beams (full block: 64 antennas, 256 beams; 256 antennas, 1024 beams),
    * 128- or 64-beam classic tiles below; FP6 runs shared tiles only */
-  d->variant = c->nbeam > 128 || c->precision == DSAXBF_PREC_FP6SPLIT ? BFK_VAR_SHARED_256
+  d->variant = c->nbeam > 128 || c->precision >= DSAXBF_PREC_FP6SPLIT ? BFK_VAR_SHARED_256
                : c->nbeam > 64 ? (d->nkc == 1 ? BFK_VAR_CLASSIC_128S : BFK_VAR_CLASSIC_128)
                                : BFK_VAR_CLASSIC_64;
 }
@@ -169,6 +169,99 @@ int dsaxbf_set_weights(dsaxbf_plan *p, const float *antpos, const float *cal,
   return rc;
 }
 
+/* ---- the weights as the kernels apply them ----------------------------- */
+static double f16_value(uint16_t h) {
+  const int e = (h >> 10) & 31, m = h & 1023;
+  const double v = e == 0 ? ldexp((double)m, -24) : ldexp(1024.0 + m, e - 25);
+  return (h & 0x8000u) ? -v : v;
+}
+static double e4m3_value(uint8_t c) {
+  const int e = (c >> 3) & 15, m = c & 7;
+  const double v = e == 0 ? ldexp((double)m, -9) : ldexp(8.0 + m, e - 10);
+  return (c & 0x80u) ? -v : v;
+}
+static double e2m3_value(uint32_t c) {
+  const int e = (c >> 3) & 3, m = c & 7;
+  const double v = e == 0 ? ldexp((double)m, -3) : ldexp(8.0 + m, e - 4);
+  return (c & 0x20u) ? -v : v;
+}
+/* 6-bit slot j of an FP6 operand's six dwords */
+static uint32_t f6_slot(const uint32_t *dw, int j) {
+  const int bit = 6 * j, k = bit >> 5, sh = bit & 31;
+  uint32_t v = dw[k] >> sh;
+  if (sh > 26) v |= dw[k + 1] << (32 - sh);
+  return v & 63u;
+}
+
+int dsaxbf_get_weights(dsaxbf_plan *p, double *w) {
+  dsaxbf_clear_error();
+  if (!p || !w) return dsaxbf_fail(DSAXBF_EINVAL, "NULL argument");
+  if (!p->have_weights) return dsaxbf_fail(DSAXBF_ESTATE, "plan has no weights");
+  const bfk_dims *d = &p->dims;
+  HIPCHK(hipSetDevice(p->cfg.device), "hipSetDevice");
+  const size_t nf = bfk_wfrag_bytes(d), ns = bfk_wscale_bytes(d);
+  uint8_t *frag = malloc(nf);
+  float *wsc = malloc(ns);
+  if (!frag || !wsc) {
+    free(frag);
+    free(wsc);
+    return dsaxbf_fail(DSAXBF_ENOMEM, "host allocation failed");
+  }
+  hipError_t e = hipMemcpy(frag, p->d_wfrag, nf, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(wsc, p->d_wscale, ns, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    free(frag);
+    free(wsc);
+    return dsaxbf_fail(DSAXBF_EHIP, "weights download: %s", hipGetErrorString(e));
+  }
+  /* voltage code unit folded into the power factor: FP16 1, FP8 2^-9, FP6 1/8 */
+  const double vunit = d->prec == 0 ? 1.0 : d->prec == 1 ? ldexp(1.0, -9) : 0.125;
+  const int nparts = d->prec == 1 || d->prec == 2 ? 2 : 3;
+  for (int cq = 0; cq < d->ncal * 2; cq++)
+    for (int b = 0; b < d->nbeam; b++) {
+      const int bt = b >> 5, n = b & 31;
+      const double f = sqrt((double)wsc[(size_t)cq * d->nbeam_pad + b]) * vunit;
+      double *row = w + ((size_t)cq * d->nbeam + b) * d->nant * 2;
+      for (int a = 0; a < d->nant; a++) {
+        const int kc = a >> 6, h = (a >> 5) & 1, lane = h * 32 + n, a32 = a & 31;
+        double re, im;
+        if (d->prec == 0) {
+          const int s = a32 >> 2, i = a32 & 3;   /* slots [r0 r1 i0 i1 r2 r3 i2 i3] */
+          const size_t rec = ((((size_t)cq * d->nbt + bt) * d->nkc + kc) * 8 + s) * 64 + lane;
+          const uint16_t *v = (const uint16_t *)(frag + rec * 16);
+          const int sr = (i >> 1) * 4 + (i & 1);
+          re = f16_value(v[sr]);
+          im = f16_value(v[sr + 2]);
+        } else {
+          const int s2 = a32 >> 4, j = a32 & 15;
+          const size_t rec = ((((size_t)cq * d->nbt + bt) * d->nkc + kc) * 2 + s2) * nparts * 64 + lane;
+          const uint32_t *hi = (const uint32_t *)(frag + rec * 32);
+          if (d->prec == 1) {
+            const uint32_t *lo = (const uint32_t *)(frag + (rec + 64) * 32);
+            const int u = j >> 2, sh = 8 * (j & 3);
+            re = e4m3_value((uint8_t)(hi[2 * u] >> sh)) + e4m3_value((uint8_t)(lo[2 * u] >> sh));
+            im = e4m3_value((uint8_t)(hi[2 * u + 1] >> sh)) + e4m3_value((uint8_t)(lo[2 * u + 1] >> sh));
+          } else {
+            /* parts 0..nparts-1 with E8M0 scales: hi dwords 6, 7; mid dword 6 */
+            re = im = 0.0;
+            for (int pt = 0; pt < nparts; pt++) {
+              const uint32_t *dw = (const uint32_t *)(frag + (rec + 64 * pt) * 32);
+              const uint32_t sexp = pt < 2 ? hi[6 + pt] : ((const uint32_t *)(frag + (rec + 64) * 32))[6];
+              const double s = ldexp(1.0, (int)sexp - 127);
+              re += e2m3_value(f6_slot(dw, j)) * s;
+              im += e2m3_value(f6_slot(dw, 16 + j)) * s;
+            }
+          }
+        }
+        row[2 * a] = re * f;
+        row[2 * a + 1] = im * f;
+      }
+    }
+  free(frag);
+  free(wsc);
+  return DSAXBF_OK;
+}
+
 int dsaxbf_execute(dsaxbf_plan *p, const uint8_t *d_in, const float *d_scale, uint8_t *d_out,
                    float *d_power, float *d_beam_power, void *stream) {
   if (!p || !d_in || !d_scale || !d_out) return dsaxbf_fail(DSAXBF_EINVAL, "NULL argument");
@@ -270,7 +363,7 @@ int dsaxbf_plan_internals(dsaxbf_plan *p, bfk_dims *dims, void **d_wfrag, float 
 }
 
 int dsaxbf_time_variants(dsaxbf_plan *p, const uint8_t *d_in, const float *d_scale, uint8_t *d_out,
-                         int reps, int batchable, float *ms, int *best_out) {
+                         int reps, float *ms, int *best_out) {
   if (!p || !d_in || !d_scale || !d_out) return dsaxbf_fail(DSAXBF_EINVAL, "NULL argument");
   if (reps < 1) return dsaxbf_fail(DSAXBF_EINVAL, "reps must be >= 1");
   if (!p->have_weights && p->dims.stuff != DSAXBF_STUFF_ANTS)
@@ -287,8 +380,7 @@ int dsaxbf_time_variants(dsaxbf_plan *p, const uint8_t *d_in, const float *d_sca
   int best = keep, rc = DSAXBF_OK;
   float best_ms = -1.f;
   for (int v = 0; v < DSAXBF_NVARIANT && rc == DSAXBF_OK; v++) {
-    if (!bfk_variant_supported(&p->dims, v) ||
-        (batchable && v == BFK_VAR_PIPELINED_256)) {
+    if (!bfk_variant_supported(&p->dims, v)) {
       if (ms) ms[v] = -1.f;   /* not a candidate for this shape */
       continue;
     }
@@ -329,7 +421,7 @@ int dsaxbf_autotune(dsaxbf_plan *p, int reps, float *ms) {
   HIPCHK(hipMemsetAsync(p->d_in, 0x5a, dsaxbf_input_bytes(&p->cfg), p->stream), "hipMemsetAsync");
   HIPCHK(hipMemsetAsync(p->d_scale, 0, sizeof(float) * p->cfg.nbeam, p->stream), "hipMemsetAsync");
   int best;
-  int rc = dsaxbf_time_variants(p, p->d_in, p->d_scale, p->d_out, reps, 0, ms, &best);
+  int rc = dsaxbf_time_variants(p, p->d_in, p->d_scale, p->d_out, reps, ms, &best);
   if (rc == DSAXBF_OK) p->dims.variant = best;
   return rc;
 }

@@ -18,7 +18,7 @@
  * Extra options (reference defaults when absent):
  *   -I file   input blocks ("-" = stdin)      -O file  output blocks ("-" = stdout)
  *   -n nant   antennas per block (63)          -T tavg  samples per output (8)
- *   -P prec   fp16 | fp8split | fp6split (fp16)   -W mode  group | fine (group)
+ *   -P prec   fp16 | fp8split | fp6split | fp6split3 (fp16)   -W mode  group | fine (group)
  *   -A mode   theta | sin (theta)              -r       input is dsaX_split -r order
  *   -N n      stop after n blocks              -w       re-read weights every block
  */
@@ -48,7 +48,8 @@ static void usage(void) {
           " -t test pattern \n"
           " -h print usage\n"
           " -I infile (- = stdin)   -O outfile (- = stdout)   -n nant [63]\n"
-          " -T tavg [8]  -P fp16|fp8split|fp6split  -W group|fine  -A theta|sin  -r  -N nblocks  -w\n");
+          " -T tavg [8]  -P fp16|fp8split|fp6split|fp6split3  -W group|fine  -A theta|sin  -r  -N nblocks\n"
+          " -w\n");
 }
 
 /* Read exactly n bytes unless EOF; returns bytes read. */
@@ -106,6 +107,7 @@ int main(int argc, char *argv[]) {
         if (strcmp(optarg, "fp16") == 0) prec = DSAXBF_PREC_FP16;
         else if (strcmp(optarg, "fp8split") == 0) prec = DSAXBF_PREC_FP8SPLIT;
         else if (strcmp(optarg, "fp6split") == 0) prec = DSAXBF_PREC_FP6SPLIT;
+        else if (strcmp(optarg, "fp6split3") == 0) prec = DSAXBF_PREC_FP6SPLIT3;
         else {
           fprintf(stderr, "dsaX_beamformer: unknown precision %s\n", optarg);
           usage();
@@ -124,6 +126,15 @@ int main(int argc, char *argv[]) {
     syslog(LOG_ERR, "psrdada rings %x/%x are not available in this build; use -I and -O", in_key,
            out_key);
     fprintf(stderr, "dsaX_beamformer: no psrdada in this build; give -I infile -O outfile\n");
+    return EXIT_FAILURE;
+  }
+  if (nant < 1 || nant > 64) {
+    /* the weights file (init_weights, :559-602) holds positions and
+       calibrations of 64 antennas */
+    syslog(LOG_ERR, "-n %d: the weights file holds 64 antennas", nant);
+    fprintf(stderr, "dsaX_beamformer: -n %d outside 1..64 (the weights file holds 64 antennas)\n",
+            nant);
+    usage();
     return EXIT_FAILURE;
   }
   if (wmode == DSAXBF_WEIGHTS_FINE) {
@@ -219,6 +230,9 @@ int main(int argc, char *argv[]) {
     }
   }
 
+  /* the reference prints ANTPOL_RMS to stdout because its data goes to a
+     ring; here stdout may carry the data blocks, so the lines go to stderr */
+  FILE *frms = fout == stdout ? stderr : stdout;
   uint8_t *block = malloc(block_size), *out = malloc(block_out);
   float scale[256], beam_power[256], rms[128];
   memset(scale, 0, sizeof(scale));
@@ -236,7 +250,7 @@ int main(int argc, char *argv[]) {
     if (slow_down == 0) /* rms_printer: packet 8 of the first four ints, group 24 */
       for (int st = 0; st < 4; st++)
         if (dsaxbf_antpol_rms(&cfg, block, st * 16 + 8, 24, rms) == 0)
-          for (int i = 0; i < 128; i++) printf("ANTPOL_RMS %d %d %f\n", i % 64, i / 64, rms[i]);
+          for (int i = 0; i < 128; i++) fprintf(frms, "ANTPOL_RMS %d %d %f\n", i % 64, i / 64, rms[i]);
     rc = dsaxbf_process_block(plan, block, scale, out, NULL, beam_power);
     if (rc) {
       syslog(LOG_ERR, "beamform: %s", dsaxbf_last_error());

@@ -24,6 +24,7 @@ LIB_PATH = os.environ.get("DSAXBF_LIB") or os.path.join(os.path.dirname(_HERE), 
 PREC_FP16 = 0
 PREC_FP8SPLIT = 1
 PREC_FP6SPLIT = 2
+PREC_FP6SPLIT3 = 3
 WEIGHTS_GROUP = 0
 WEIGHTS_FINE = 1
 ANGLE_THETA = 0
@@ -34,13 +35,13 @@ STUFF_IB = 2
 ORDER_CAPTURE = 0
 ORDER_SPLIT_R = 1
 VARIANT_CLASSIC_256 = 0
-VARIANT_PIPELINED_256 = 1
+VARIANT_RETIRED = 1          # a persistent schedule that never won: no shape runs it
 VARIANT_CLASSIC_128S = 2
 VARIANT_CLASSIC_128 = 3
 VARIANT_CLASSIC_64 = 4
 VARIANT_SHARED_256 = 5
 NVARIANT = 6
-VARIANT_NAMES = {0: "classic256", 1: "pipelined256", 2: "classic128s", 3: "classic128", 4: "classic64",
+VARIANT_NAMES = {0: "classic256", 1: "retired", 2: "classic128s", 3: "classic128", 4: "classic64",
                  5: "shared256"}
 
 ERRORS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "EIO", -5: "ESTATE", -6: "ENODEV"}
@@ -97,6 +98,7 @@ def _load():
         "dsaxbf_get_stream": (P, [P]),
         "dsaxbf_read_weights_file": (C.c_int, [C.c_char_p, C.c_char_p, P, P]),
         "dsaxbf_set_weights": (C.c_int, [P, P, P, P, P, C.c_double]),
+        "dsaxbf_get_weights": (C.c_int, [P, P]),
         "dsaxbf_reference_freqs": (None, [C.c_double, C.c_int, P]),
         "dsaxbf_execute": (C.c_int, [P, P, P, P, P, P, P]),
         "dsaxbf_process_block": (C.c_int, [P, P, P, P, P, P]),
@@ -143,7 +145,7 @@ lib = _load()
 EXPORTED = [
     "dsaxbf_config_default", "dsaxbf_input_bytes", "dsaxbf_output_bytes", "dsaxbf_last_error",
     "dsaxbf_create", "dsaxbf_destroy", "dsaxbf_get_config", "dsaxbf_get_stream",
-    "dsaxbf_read_weights_file", "dsaxbf_set_weights", "dsaxbf_reference_freqs",
+    "dsaxbf_read_weights_file", "dsaxbf_set_weights", "dsaxbf_get_weights", "dsaxbf_reference_freqs",
     "dsaxbf_execute", "dsaxbf_beam_power", "dsaxbf_set_variant", "dsaxbf_get_variant",
     "dsaxbf_autotune", "dsaxbf_batch_create", "dsaxbf_batch_autotune", "dsaxbf_batch_launch",
     "dsaxbf_batch_kernels", "dsaxbf_batch_destroy", "dsaxbf_pipe_create", "dsaxbf_pipe_destroy", "dsaxbf_pipe_input",
@@ -275,6 +277,13 @@ class Plan:
         assert a.size == self.cfg.nant and c.size == self.cfg.nant * self.cfg.ncal * 4
         assert f.size == self.cfg.nchan and (t is None or t.size == self.cfg.nbeam)
         _check(lib.dsaxbf_set_weights(self._h, _ptr(a), _ptr(c), _ptr(f), _ptr(t), sep_arcmin))
+
+    def get_weights(self) -> np.ndarray:
+        """The weights as the kernels apply them: [ncal][2][nbeam][nant] complex."""
+        cfg = self.cfg
+        w = np.empty((cfg.ncal, 2, cfg.nbeam, cfg.nant, 2), np.float64)
+        _check(lib.dsaxbf_get_weights(self._h, _ptr(w)))
+        return w[..., 0] + 1j * w[..., 1]
 
     def process_block(self, block: np.ndarray, scale: np.ndarray, want_power=False,
                       want_beam_power=False):

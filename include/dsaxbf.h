@@ -63,9 +63,15 @@ extern "C" {
 enum dsaxbf_precision {
   DSAXBF_PREC_FP16    = 0,  /* FP16 weights x FP16 voltages, FP32 accumulate   */
   DSAXBF_PREC_FP8SPLIT = 1, /* FP8 E4M3 voltages x (hi+lo) FP8 weights, FP32   */
-  DSAXBF_PREC_FP6SPLIT = 2  /* FP6 E2M3 voltages x (hi+lo) FP6 weights, FP32
+  DSAXBF_PREC_FP6SPLIT = 2, /* FP6 E2M3 voltages x (hi+lo) FP6 weights, FP32
                                accumulate, at twice the FP8 matrix rate (shared
                                schedule only).  No counterpart in the
+                               reference.                                     */
+  DSAXBF_PREC_FP6SPLIT3 = 3 /* FP6 E2M3 voltages x (hi+mid+lo) FP6 weights,
+                               FP32 accumulate: three 32-cycle FP6 passes where
+                               FP16 takes two 32-cycle passes' worth, with
+                               weights good to ~12 bits (shared schedule, up
+                               to 128 antennas).  No counterpart in the
                                reference.                                     */
 };
 enum dsaxbf_weight_mode {
@@ -155,6 +161,16 @@ int dsaxbf_set_weights(dsaxbf_plan *plan, const float *antpos, const float *cal,
                        const double *freqs_hz, const double *theta_rad,
                        double sep_arcmin);
 
+/* The beam weights exactly as the kernels apply them, read back from the
+ * device and decoded on the host: the plan's precision encoding (binary16,
+ * or the FP8 / FP6 split parts with their per-lane scales) times the power
+ * factor of its row, i.e. the weights the reference's calc_weights leaves in
+ * its half-precision host arrays before they go to the GPU
+ * (dsaX_beamformer.cu:442-446, 948-951).  w: [ncal][2 pol][nbeam][nant]
+ * complex as (re, im) doubles, ncal = nchan/8 (group) or nchan (fine).
+ * For validation (an oracle fed the path's own weights); synchronous. */
+int dsaxbf_get_weights(dsaxbf_plan *plan, double *w);
+
 /* Reference channel frequencies: freqs[i] = (fch1_mhz - i*250/8192)*1e6
  * (dsaX_beamformer.cu:799-800). */
 void dsaxbf_reference_freqs(double fch1_mhz, int nchan, double *freqs_hz);
@@ -187,9 +203,6 @@ int dsaxbf_beam_power(dsaxbf_plan *plan, float *d_beam_power, void *stream);
  * favour small tiles) and the GPU.
  *   CLASSIC_256    one tile (256 beams x 64 samples x 8 channels) per
  *                  workgroup, two workgroups per CU (nant <= 64);
- *   PIPELINED_256  one persistent workgroup per CU with the next tile's
- *                  staging and the next weights' loads overlapped with the MFMA
- *                  work (nant <= 64);
  *   CLASSIC_128S   128-beam tiles: 64 beams per wave, wave pairs split the
  *                  tile's samples (nant <= 64; default for 65-128 beams);
  *   CLASSIC_128    128-beam tiles: 32 beams per wave (default for 65-128
@@ -199,11 +212,12 @@ int dsaxbf_beam_power(dsaxbf_plan *plan, float *d_beam_power, void *stream);
  *   SHARED_256     256-beam tiles whose waves unpack each unit once into LDS
  *                  operands they all read: 4 waves of 64 beams (nant <= 64) or
  *                  8 waves of 32 beams (nant > 64) (default above 128 beams).
+ * Value 1 is retired (a persistent schedule that never won): no shape
+ * supports it, and dsaxbf_set_variant rejects it.
  * No counterpart in the reference (its five kernels per int have one fixed
  * launch shape, dsaX_beamformer.cu:955-1003). */
 enum {
   DSAXBF_VARIANT_CLASSIC_256 = 0,
-  DSAXBF_VARIANT_PIPELINED_256 = 1,
   DSAXBF_VARIANT_CLASSIC_128S = 2,
   DSAXBF_VARIANT_CLASSIC_128 = 3,
   DSAXBF_VARIANT_CLASSIC_64 = 4,
@@ -291,6 +305,8 @@ int dsaxbf_synth_geometry(int nant, int ncal, int nbeam, uint64_t seed,
  * per-block beam-power sums) are captured once into a HIP graph, which
  * dsaxbf_batch_launch replays.  The plans must outlive the batch; their
  * variants at creation (or after dsaxbf_batch_autotune) fix the tile shapes.
+ * Each block needs a plan of its own (a plan holds its block's beam-power
+ * scratch): the same plan twice is DSAXBF_EINVAL.
  * No counterpart in the reference (one fixed-shape block per process). */
 typedef struct dsaxbf_batch dsaxbf_batch;
 int dsaxbf_batch_create(dsaxbf_batch **batch, dsaxbf_plan *const *plans, int n,

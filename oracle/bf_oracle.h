@@ -81,6 +81,28 @@ typedef struct orc_agc {
 void orc_agc_init(orc_agc *s, int nbeam, int ngroup, int nints, int enabled);
 int  orc_agc_update(orc_agc *s, const float *bp_raw, float *scale);
 
+/* ---- weight encodings of the GPU path (bf_quant.c) -------------------- */
+/* Code values and round-to-nearest-even encoders, saturating at the largest
+ * finite value (binary16: overflow to infinity): OCP E4M3, OCP MX E2M3
+ * (6-bit code), IEEE binary16. */
+double   orc_e4m3_decode(uint32_t c);
+uint32_t orc_e4m3_encode(double v);
+double   orc_e2m3_decode(uint32_t c);
+uint32_t orc_e2m3_encode(double v);
+double   orc_f16_decode(uint32_t h);
+uint32_t orc_f16_encode(double v);
+/* Weights (orc_weights layout) through the GPU path's encoding of precision
+ * prec, as the values its matrix cores multiply (scales undone):
+ *   0 FP16: w*2^e rounded to binary16, the row's largest component in
+ *     [2^14, 2^15) (calc_weights' __float2half, dsaX_beamformer.cu:442-446);
+ *   1 FP8 split: hi = E4M3(w*2^e), lo = E4M3(w*2^e - hi), largest in [128, 256);
+ *   2 FP6 split: x = w*7.5/max; per 16 antennas (32 components) a part is
+ *     E2M3 of the remainder boosted by the largest power of two keeping it
+ *     within 7.5; hi and lo parts;
+ *   3 FP6 three-part split: hi, mid and lo parts that way.
+ * Returns 0, or < 0 for an unknown precision or no memory. */
+int orc_quant_weights(const orc_dims *d, int wmode, int prec, const double *W, double *Wq);
+
 /* ---- FP32 multithreaded CPU baseline (same op order, float) ---------- */
 /* Weights as float complex in the same layout as orc_weights; writes float
  * power and uint8 output; returns wall seconds.  nthreads <= 0: all cores. */

@@ -49,6 +49,13 @@ def lib():
         L.orc_agc_init.argtypes = [C.POINTER(AgcState), C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_agc_update.restype = C.c_int
         L.orc_agc_update.argtypes = [C.POINTER(AgcState), P, P]
+        for fmt in ("e4m3", "e2m3", "f16"):
+            getattr(L, f"orc_{fmt}_decode").restype = C.c_double
+            getattr(L, f"orc_{fmt}_decode").argtypes = [C.c_uint32]
+            getattr(L, f"orc_{fmt}_encode").restype = C.c_uint32
+            getattr(L, f"orc_{fmt}_encode").argtypes = [C.c_double]
+        L.orc_quant_weights.restype = C.c_int
+        L.orc_quant_weights.argtypes = [C.POINTER(Dims), C.c_int, C.c_int, P, P]
         L.cpu_beamform.restype = C.c_double
         L.cpu_beamform.argtypes = [C.POINTER(Dims), C.c_int, P, P, P, C.c_int, C.c_int, P, P,
                                    C.c_int]
@@ -104,6 +111,31 @@ def power(d: Dims, wmode: int, block, W, stuff=0, g0=0, g1=None) -> np.ndarray:
     return out
 
 
+def decode(fmt: str, code: int) -> float:
+    """Value of a code of format fmt ("e4m3", "e2m3", "f16")."""
+    return float(getattr(lib(), f"orc_{fmt}_decode")(code))
+
+
+def encode(fmt: str, v: float) -> int:
+    """Round-to-nearest-even code of v in format fmt ("e4m3", "e2m3", "f16")."""
+    return int(getattr(lib(), f"orc_{fmt}_encode")(v))
+
+
+def quant_weights(d: Dims, wmode: int, prec: int, W) -> np.ndarray:
+    """Weights W ([ncal][2][nbeam][nant] complex) through the GPU path's
+    encoding of precision prec (0 FP16, 1 FP8 split, 2 FP6 split, 3 FP6
+    three-part split), as the values its matrix cores multiply."""
+    ncal = d.nchan // 8 if wmode == 0 else d.nchan
+    if W.shape != (ncal, 2, d.nbeam, d.nant):
+        raise ValueError(f"weights of shape {W.shape}, want {(ncal, 2, d.nbeam, d.nant)}")
+    Wr = np.ascontiguousarray(np.stack([W.real, W.imag], -1), np.float64)
+    Wq = np.empty_like(Wr)
+    rc = lib().orc_quant_weights(C.byref(d), wmode, prec, _p(Wr), _p(Wq))
+    if rc:
+        raise ValueError(f"orc_quant_weights: error {rc}")
+    return Wq[..., 0] + 1j * Wq[..., 1]
+
+
 def requant(x: float, scale: float) -> int:
     return int(lib().orc_requant(x, scale))
 
@@ -137,8 +169,31 @@ class Agc:
         return sc, bool(z)
 
 
-def cpu_beamform(d: Dims, wmode: int, block, W, scale, g0=0, g1=None, nthreads=0):
-    """FP32 multithreaded CPU baseline; returns (seconds, power, out)."""
+def native_cpu_lib(outdir: str):
+    """The CPU baseline (bf_cpu.c with the oracle it calls) compiled with
+    -O3 -march=native for the host this runs on, as a separate library in
+    outdir; None when no C compiler is available."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        return None
+    here = os.path.dirname(LIB_PATH)
+    so = os.path.join(outdir, "liborc_native.so")
+    cmd = [cc, "-O3", "-march=native", "-fPIC", "-std=gnu11", "-pthread", "-shared", "-o", so,
+           os.path.join(here, "bf_oracle.c"), os.path.join(here, "bf_quant.c"),
+           os.path.join(here, "bf_cpu.c"), "-lm", "-lpthread"]
+    if subprocess.run(cmd, capture_output=True).returncode != 0:
+        return None
+    L = C.CDLL(so)
+    L.cpu_beamform.restype = C.c_double
+    L.cpu_beamform.argtypes = lib().cpu_beamform.argtypes
+    return L
+
+
+def cpu_beamform(d: Dims, wmode: int, block, W, scale, g0=0, g1=None, nthreads=0, native=None):
+    """FP32 multithreaded CPU baseline; returns (seconds, power, out).
+    native: a library from native_cpu_lib() to run instead of liborc."""
     ng = d.nchan // 8
     g1 = ng if g1 is None else g1
     Wf = np.ascontiguousarray(np.stack([W.real, W.imag], -1), np.float32)
@@ -146,6 +201,6 @@ def cpu_beamform(d: Dims, wmode: int, block, W, scale, g0=0, g1=None, nthreads=0
     pw = np.zeros(shape, np.float32)
     out = np.zeros(shape, np.uint8)
     sc = np.ascontiguousarray(scale, np.float32)
-    secs = lib().cpu_beamform(C.byref(d), wmode, _p(np.ascontiguousarray(block, np.uint8)),
+    secs = (native or lib()).cpu_beamform(C.byref(d), wmode, _p(np.ascontiguousarray(block, np.uint8)),
                               _p(Wf), _p(sc), g0, g1, _p(pw), _p(out), nthreads)
     return secs, pw, out

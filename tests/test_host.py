@@ -59,10 +59,12 @@ def test_config_defaults_match_reference(bf):
 
 @pytest.mark.parametrize("field,value", [
     ("nant", 0), ("nant", 257), ("nchan", 12), ("npkt", 48), ("nbeam", 0), ("nbeam", 1025),
-    ("tavg", 4), ("tavg", 24), ("precision", 3), ("weight_mode", 3), ("angle_mode", -1),
-    ("stuffants", 3), ("input_order", 2)])
+    ("tavg", 4), ("tavg", 24), ("precision", 4), ("precision", -1), ("weight_mode", 3),
+    ("angle_mode", -1), ("stuffants", 3), ("input_order", 2), ("nant", 129)])
 def test_create_rejects_bad_config(bf, field, value):
-    cfg = bf.config(nant=16, nchan=16, npkt=64, nbeam=32)
+    # nant 129 is rejected for the three-part FP6 split only (up to 128 antennas)
+    cfg = bf.config(nant=16, nchan=16, npkt=64, nbeam=32,
+                    precision=bf.PREC_FP6SPLIT3 if (field, value) == ("nant", 129) else bf.PREC_FP16)
     setattr(cfg, field, value)
     with pytest.raises(bf.BeamformerError) as e:
         bf.Plan(cfg)
@@ -198,6 +200,16 @@ def test_cli_usage_and_missing_io():
     assert r.returncode == 0 and "-z fch1 in MHz [default 1530]" in r.stdout
     r = subprocess.run([exe, "-f", "nofile"], capture_output=True, text=True)
     assert r.returncode != 0 and "psrdada" in r.stderr
-    assert "fp16|fp8split|fp6split" in subprocess.run([exe, "-h"], capture_output=True, text=True).stdout
+    assert "fp16|fp8split|fp6split|fp6split3" in subprocess.run([exe, "-h"], capture_output=True,
+                                                                text=True).stdout
     r = subprocess.run([exe, "-P", "fp4", "-I", "-", "-O", "-"], capture_output=True, text=True)
     assert r.returncode != 0 and "unknown precision fp4" in r.stderr
+
+
+@pytest.mark.parametrize("nant", ["65", "256", "0"])
+def test_cli_rejects_antenna_counts_beyond_the_weights_file(nant):
+    # the weights file holds 64 antennas: larger -n is an error before any GPU call
+    exe = os.path.join(ROOT, "dsa110-xengine_amd", "bin", "dsaX_beamformer")
+    r = subprocess.run([exe, "-n", nant, "-I", "-", "-O", "-"], capture_output=True, text=True,
+                       input="")
+    assert r.returncode != 0 and "outside 1..64" in r.stderr

@@ -224,6 +224,36 @@ def test_power_single_antenna_and_linearity(orc, bf):
     assert (orc.power(d, 1, np.zeros_like(block).ravel(), W) == 0).all()
 
 
+def test_beam_power_sums_outputs_and_groups(orc, bf):
+    """orc_beam_power (badder + calc_allbp, dsaX_beamformer.cu:78-112,482-495)
+    on a shape where nout (8), nbeam (5) and ngroup (3) all differ: the sum of
+    the power over outputs and groups, and for one live antenna the closed
+    form sum_{t,c,q} |W[c,q,b,a]|^2 |d[t,c,q]|^2."""
+    nant, nchan, npkt, nbeam, tavg = 12, 24, 64, 5, 16
+    cfg = bf.config(nant=nant, nchan=nchan, npkt=npkt, nbeam=nbeam, tavg=tavg)
+    antpos, cal, theta = _geom(bf, nant, nchan, nbeam)
+    d = orc.dims(nant, nchan, npkt, nbeam, tavg)
+    W = orc.weights(d, 1, 1, antpos, cal, bf.reference_freqs(1450.0, nchan), theta)
+    block = bf.synth_block(cfg, seed=5).reshape(npkt, nant, nchan, 2, 2)
+    P = orc.power(d, 1, block.ravel(), W)
+    assert P.shape == (8, nbeam, 3)
+    assert np.allclose(orc.beam_power(d, P), P.sum(axis=(0, 2)), rtol=1e-13)
+    one = np.zeros_like(block)
+    one[:, 7] = block[:, 7]
+    P1 = orc.power(d, 1, one.ravel(), W)
+    raw = one[:, 7].astype(np.int64)
+    lo, hi = raw & 15, raw >> 4
+    e2 = np.where(lo >= 8, lo - 16, lo) ** 2 + np.where(hi >= 8, hi - 16, hi) ** 2   # p c t q
+    w2 = np.abs(W[:, :, :, 7]) ** 2                                                  # c q b
+    expect = np.einsum("pctq,cqb->b", e2, w2)
+    assert np.allclose(orc.beam_power(d, P1), expect, rtol=1e-12)
+    # constant voltage (re = 1, im = 0 on every antenna): |B|^2 = |sum_a W|^2
+    ones = np.full_like(block, 0x01)
+    Pc = orc.power(d, 1, ones.ravel(), W)
+    expect_c = 2 * npkt * np.einsum("cqb->b", np.abs(W.sum(axis=3)) ** 2)
+    assert np.allclose(orc.beam_power(d, Pc), expect_c, rtol=1e-12)
+
+
 def test_power_steered_tone_peaks_at_its_beam(orc, bf):
     nant, nchan, npkt, nbeam = 32, 8, 64, 33
     cfg = bf.config(nant=nant, nchan=nchan, npkt=npkt, nbeam=nbeam, tavg=32)
