@@ -322,6 +322,53 @@ def stream_bench(dsaxbf, ev, nblocks=8, seed=77):
     }
 
 
+def xengine_bench(dsaxbf, ev, reps=20, seed=91):
+    """The correlator X-engine on a full block of the reference correlator's
+    shape: 96 antennas x 384 channels x 4096 samples (2048 packets, 302 MB
+    packed), all 4656 baselines x 2 pols, visibilities exact.  Kernel time
+    from HIP events on the plan's stream over reps launches, three distinct
+    blocks in rotation (larger together than the last-level cache)."""
+    nant, nchan, npkt = 96, 384, 2048
+    cfg = dsaxbf.config(nant=nant, nchan=nchan, npkt=npkt, nbeam=8)
+    blocks = [dsaxbf.synth_block(cfg, seed + k) for k in range(3)]
+    x = dsaxbf.XEngine(nant, nchan, npkt)
+    d_in = [dsaxbf.DeviceBuffer(b.nbytes) for b in blocks]
+    for buf, b in zip(d_in, blocks):
+        buf.upload(b)
+    d_vis = dsaxbf.DeviceBuffer(x.vis_bytes)
+    stream = x.stream
+    for i in range(3):
+        x.execute(d_in[i], d_vis, stream)
+    ev.sync()
+    marks = ev.create(2 * reps)
+    for i in range(reps):
+        ev.record(marks[2 * i], stream)
+        x.execute(d_in[i % 3], d_vis, stream)
+        ev.record(marks[2 * i + 1], stream)
+    ms = [ev.ms(marks[2 * i], marks[2 * i + 1]) for i in range(reps)]
+    kern_ms = float(np.mean(ms))
+    nin = blocks[0].nbytes
+    alg_bytes = nin + x.vis_bytes
+    nbase = nant * (nant + 1) // 2
+    # complex MACs of the triangle (4 real MACs each), and the FP6 matrix work
+    # issued: 6 32x32 tiles x 2 (re, im) per 32 samples per (channel, pol)
+    flop_alg = 8.0 * nbase * nchan * 2 * 2 * npkt
+    flop_exec = 2.0 * 32 * 32 * 64 * 6 * 2 * (2 * npkt // 32) * nchan * 2
+    x.close()
+    for b in d_in + [d_vis]:
+        b.free()
+    return {"workload": "96 ant x 2 pol x 384 ch x 4096 samples (302 MB packed, capture order) -> "
+                        "4656 baselines x 384 ch x 2 pols, exact integer visibilities",
+            "kernel": "xc_kernel<3>", "kernel_ms": kern_ms, "kernel_ms_min": float(np.min(ms)),
+            "samples_per_s": nant * 2 * nchan * 2 * npkt / (kern_ms / 1e3),
+            "roofline": {"bound": "hbm", "achieved": alg_bytes / (kern_ms / 1e3) / 1e9, "peak": 8000.0,
+                         "unit": "GB/s", "frac": alg_bytes / (kern_ms / 1e3) / 1e9 / 8000.0,
+                         "bytes_per_launch": alg_bytes},
+            "mfma_fp6_executed_frac": flop_exec / (kern_ms / 1e3) / 1e12 / PEAK_TFLOPS["fp6split"],
+            "algorithmic_tflops": flop_alg / (kern_ms / 1e3) / 1e12,
+            "realtime_fraction": kern_ms / 134.217728}
+
+
 def batch_bench(dsaxbf, ev, seed=2025, nblocks=24, reps=5):
     """Uneven blocks in one graph launch: nblocks blocks of 64 antennas with
     channel counts from {48, 96, 192, 384}, beam counts from {64, 128, 256, 512}
@@ -330,9 +377,7 @@ def batch_bench(dsaxbf, ev, seed=2025, nblocks=24, reps=5):
     each precision.  Reported: the graph replay time of the batch with the
     autotuned tile shapes, against the same blocks run one execute at a time
     with the default shape."""
-    rng = np.random.default_rng(seed)
-    shapes = [(int(rng.choice([48, 96, 192, 384])), int(rng.choice([64, 128, 256, 512])),
-               int(rng.choice([1024, 2048, 4096]))) for _ in range(nblocks)]
+    shapes = dsaxbf.uneven_batch_shapes(seed, nblocks)
     res = {"blocks": [{"nchan": c, "nbeam": b, "nsamp": t} for c, b, t in shapes]}
     flop = sum(8.0 * b * 64 * c * 2 * t for c, b, t in shapes)
     samples = sum(64 * 2 * c * t for c, b, t in shapes)
@@ -415,6 +460,8 @@ def main():
                     help="skip the batched uneven-blocks measurement (one graph launch)")
     ap.add_argument("--no-stream", action="store_true",
                     help="skip the streamed scaled-array measurement (256 antennas, 1024 beams)")
+    ap.add_argument("--no-xengine", action="store_true",
+                    help="skip the correlator X-engine measurement (96 antennas, full block)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy, three times: the "
                          "known-size kernel that scales the HBM byte counters")
@@ -475,6 +522,14 @@ def main():
             stream = {"error": f"{type(exc).__name__}: {exc}"}
         print(f"# stream: {stream}", file=sys.stderr, flush=True)
 
+    xeng = None
+    if not args.no_xengine:
+        try:
+            xeng = xengine_bench(dsaxbf, ev)
+        except Exception as exc:             # reported, never fatal to the headline line
+            xeng = {"error": f"{type(exc).__name__}: {exc}"}
+        print(f"# xengine: {xeng}", file=sys.stderr, flush=True)
+
     batch = None
     if not args.no_batch:
         try:
@@ -525,6 +580,7 @@ def main():
         "tone_check": {k: errs[k] for k in ("tone_beam", "peak_beam_fp16") if k in errs},
         "scaled_array_stream": stream,
         "uneven_batch_graph": batch,
+        "correlator_xengine": xeng,
     }
     print(json.dumps(line), flush=True)
 

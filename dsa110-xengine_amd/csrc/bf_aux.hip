@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void bf_stuff_kernel(bfk_dims d, const uint8_t
     const int ant = ((b / 16) % 4) * 16 + b % 16;
     for (int k = 0; k < nbin; k++) {
       const float x = d.stuff == 2 ? IB[k] : (ant < d.nant ? S[ant][k] : 0.f);
-      const long idx = ((long)(t64 * nbin + k) * d.nbeam + b) * d.ngroup + g;
+      const long idx = out_index(d.layout, d.nout, d.nbeam, d.ngroup, t64 * nbin + k, b, g);
       if (power) power[idx] = x;
       out[idx] = requant_u8(x, scale[b]);
     }
@@ -76,12 +76,9 @@ __global__ __launch_bounds__(256) void bf_stuff_kernel(bfk_dims d, const uint8_t
 // independent partial sums (all loads of a round issued before the adds),
 // the waves reduce with cross-lane swaps and the four wave sums are added in
 // LDS, all in a fixed order, so the result is deterministic.
-__global__ __launch_bounds__(256) void bf_beam_power_kernel(bfk_dims d, const float *__restrict__ bpart,
-                                                            float *__restrict__ bp) {
+__device__ __forceinline__ void beam_sum(const float *__restrict__ src, int n, float *__restrict__ dst) {
   __shared__ float red[4];
-  const int b = blockIdx.x, tid = threadIdx.x;
-  const int n = (d.nsamp / 32) * d.ngroup;
-  const float *src = bpart + (long)b * n;
+  const int tid = threadIdx.x;
   float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   int i = tid;
   for (; i + 7 * 256 < n; i += 8 * 256) {
@@ -97,7 +94,34 @@ __global__ __launch_bounds__(256) void bf_beam_power_kernel(bfk_dims d, const fl
   for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m);
   if ((tid & 63) == 0) red[tid >> 6] = s;
   __syncthreads();
-  if (tid == 0) bp[b] = (red[0] + red[1]) + (red[2] + red[3]);
+  if (tid == 0) *dst = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(256) void bf_beam_power_kernel(bfk_dims d, const float *__restrict__ bpart,
+                                                            float *__restrict__ bp) {
+  const int n = (d.nsamp / 32) * d.ngroup;
+  beam_sum(bpart + (long)blockIdx.x * n, n, bp + blockIdx.x);
+}
+
+// The beam-power sums of every block of a batch in one launch: workgroup B
+// is beam B - beam_start[j] of job j, reduced exactly as bf_beam_power_kernel
+// does, so the sums are bit-identical to a per-block launch.
+__global__ __launch_bounds__(256) void bf_beam_power_batch_kernel(const bfk_bp_job *__restrict__ jobs,
+                                                                  const int *__restrict__ beam_start) {
+  const int B = blockIdx.x;
+  int j = 0;
+  while (B >= beam_start[j + 1]) j++;
+  const bfk_bp_job &job = jobs[j];
+  const int b = B - beam_start[j];
+  beam_sum(job.bpart + (long)b * job.n, job.n, job.out + b);
+}
+
+extern "C" int bfk_beam_power_batch(const void *d_jobs, const int *d_beam_start, int nbeams_total,
+                                    void *stream) {
+  if (nbeams_total < 1) return 0;
+  hipLaunchKernelGGL(bf_beam_power_batch_kernel, dim3(nbeams_total), dim3(256), 0, (hipStream_t)stream,
+                     (const bfk_bp_job *)d_jobs, d_beam_start);
+  return bfk_check(hipGetLastError(), "bf_beam_power_batch_kernel");
 }
 
 extern "C" int bfk_stuff(const bfk_dims *d, const uint8_t *d_in, const float *d_scale,

@@ -174,9 +174,19 @@ struct BfArgs {
   int split;      // classic: wave pairs split a tile's two 32-sample units
   int nant, nchan, ngroup, nbeam, nbeam_pad, nbt, wmode, order;
   int ntblk, nbsets, ntiles, tiles_per_xcd;
+  int layout, nout;                       // output layout (0 [t][beam][g], 1 [beam][t][g]), bins
+  int kind;                               // tile schedule of a mixed batch job (MIX_*)
   uint32_t pstride, astride, gstride;     // input bytes per packet, antenna, channel group
   uint32_t in_bytes, wfrag_bytes, wscale_bytes;
 };
+
+// Output (and power) index of bin o, beam b, group g: [o][beam][g], the
+// BF_BLOCK_KEY layout (dsaX_beamformer.cu:150-156), or beam-major
+// [beam][o][g], in which beams 64s..64s+63 are byte for byte the buffer that
+// dsaX_dbnic's transmit thread s builds by its transpose (dsaX_dbnic.c:111-116).
+__device__ __forceinline__ long out_index(int layout, int nout, int nbeam, int ngroup, int o, int b, int g) {
+  return layout == 0 ? ((long)o * nbeam + b) * ngroup + g : ((long)b * nout + o) * ngroup + g;
+}
 
 // Tile L -> (time block, group, beam set).  Consecutive tiles run at the same
 // time on one XCD.  When ngroup is a multiple of 4 the order is (group mod 4)
@@ -646,14 +656,14 @@ __device__ __forceinline__ void tile_output(const BfArgs &a, const TileId &t, in
 #pragma unroll
         for (int j = 0; j < NP; j++) {
           const int o = t.tblk * (TCU / TAVG) + tt * (32 / TAVG) + j;
-          const long idx = ((long)o * a.nbeam + b) * a.ngroup + t.g;
+          const long idx = out_index(a.layout, a.nout, a.nbeam, a.ngroup, o, b, t.g);
           if (a.power) a.power[idx] = x[k][j];
           a.out[idx] = requant_u8(x[k][j], a.scale[b]);
         }
     }
     if (TAVG == 64 && own) {
       const float y = x[0][0] + x[1][0];
-      const long idx = ((long)t.tblk * a.nbeam + b) * a.ngroup + t.g;
+      const long idx = out_index(a.layout, a.nout, a.nbeam, a.ngroup, t.tblk, b, t.g);
       if (a.power) a.power[idx] = y;
       a.out[idx] = requant_u8(y, a.scale[b]);
     }

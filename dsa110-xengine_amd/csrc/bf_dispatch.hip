@@ -45,6 +45,8 @@ BfArgs bf_make_args(const bfk_dims *d, const uint8_t *d_in, const void *d_wfrag,
   a.ntblk = d->nsamp / TCU;
   a.nbsets = (d->nbeam + bset_beams - 1) / bset_beams;   // beams past nbeam: zero weights, no output
   a.ntiles = a.nbsets * d->ngroup * a.ntblk;
+  a.layout = d->layout;
+  a.nout = d->nout;
   a.tiles_per_xcd = (a.ntiles + 7) / 8;
   a.pstride = (uint32_t)d->nant * d->nchan * 4;
   a.astride = d->order == 0 ? (uint32_t)d->nchan * 4 : 32u;

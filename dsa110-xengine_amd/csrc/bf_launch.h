@@ -32,6 +32,7 @@ typedef struct bfk_dims {
   int order;      /* 0 capture, 1 split -r                                */
   int stuff;      /* 0 none, 1 antennas, 2 incoherent beam 0              */
   int variant;    /* fused-kernel variant, BFK_VAR_*                      */
+  int layout;     /* output layout: 0 [t][beam][g], 1 [beam][t][g]        */
 } bfk_dims;
 
 /* Variants of the fused beamformer (same bits, different speed):
@@ -88,7 +89,16 @@ typedef struct bfk_batch_bufs {
   uint8_t *d_out;
   float *d_power;       /* optional */
   float *d_bpart;
+  float *d_beam_power;  /* optional: the block's per-beam power sums */
 } bfk_batch_bufs;
+
+/* One block's beam-power reduction in a batch: out[b] = sum of the n
+ * partial sums bpart[b*n .. b*n + n-1] for b < nbeam. */
+typedef struct bfk_bp_job {
+  const float *bpart;
+  float *out;
+  int nbeam, n;
+} bfk_bp_job;
 int bfk_batch_prepare(const bfk_dims *dims, const bfk_batch_bufs *bufs, int n, void **handle);
 int bfk_batch_launch(void *handle, void *stream);
 int bfk_batch_launches(void *handle);   /* kernel launches per bfk_batch_launch */

@@ -4,11 +4,13 @@
  *
  * The reference beamforms one compiled-in block shape per process
  * (dsaX_beamformer.cu:939-1090).  A batch binds n plans (each its own shape,
- * weights and device buffers) and replays, per launch, one beamforming kernel
- * per kernel instantiation in use (all tiles of all blocks sharing it) and one
- * beam-power sum per block that asks for it, all captured once in a graph.
- * The autotuner picks each block shape's variant by timing the classic and
- * shared ones on that block's own buffers.
+ * weights and device buffers) and replays, per launch, the beamforming
+ * launches of bf_jobs.hip (one launch per precision and tavg for
+ * the 64-antenna FP16 / FP8 blocks, whatever their tile shapes) and one launch
+ * for every block's beam-power sums, all captured once in a graph.  The
+ * autotuner times each block shape's variants on that block's own buffers,
+ * then lets the whole graph choose between those picks, the plans' previous
+ * variants and the shared schedule everywhere.
  */
 #include <hip/hip_runtime_api.h>
 #include <stdlib.h>
@@ -50,6 +52,7 @@ static int build(dsaxbf_batch *b) {
     if (rc) return rc;
     if (b->dims[i].stuff != DSAXBF_STUFF_NONE)
       return dsaxbf_fail(DSAXBF_EINVAL, "block %d: -s/-q modes are not batched", i);
+    b->bufs[i].d_beam_power = b->d_bp[i];
   }
   if (bfk_batch_prepare(b->dims, b->bufs, b->n, &b->kern))
     return dsaxbf_fail(DSAXBF_EINVAL, "%s", bfk_last_error());
@@ -57,11 +60,6 @@ static int build(dsaxbf_batch *b) {
   if (e != hipSuccess) return dsaxbf_fail(DSAXBF_EHIP, "begin capture: %s", hipGetErrorString(e));
   int rc = bfk_batch_launch(b->kern, b->stream);
   int nk = bfk_batch_launches(b->kern);
-  for (int i = 0; i < b->n && rc == 0; i++)
-    if (b->d_bp[i]) {
-      rc = bfk_beam_power(&b->dims[i], b->bufs[i].d_bpart, b->d_bp[i], b->stream);
-      nk++;
-    }
   hipGraph_t g = NULL;
   e = hipStreamEndCapture(b->stream, &g);
   if (rc) {
@@ -167,28 +165,76 @@ static int same_shape(const bfk_dims *x, const bfk_dims *y) {
          x->order == y->order;
 }
 
+/* Mean ms of one replay of the current graph over reps replays (after one). */
+static int time_graph(dsaxbf_batch *b, int reps, float *out_ms) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return dsaxbf_fail(DSAXBF_EHIP, "hipEventCreate");
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return dsaxbf_fail(DSAXBF_EHIP, "hipEventCreate");
+  }
+  hipError_t e = hipGraphLaunch(b->exec, b->stream);
+  if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = hipGraphLaunch(b->exec, b->stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (e != hipSuccess) return dsaxbf_fail(DSAXBF_EHIP, "batch timing: %s", hipGetErrorString(e));
+  *out_ms = t / (float)reps;
+  return DSAXBF_OK;
+}
+
+/* Set the plans' variants to v[] and capture the graph again. */
+static int apply(dsaxbf_batch *b, const int *v) {
+  for (int i = 0; i < b->n; i++) {
+    int rc = dsaxbf_set_variant(b->plans[i], v[i]);
+    if (rc) return rc;
+  }
+  return build(b);
+}
+
 int dsaxbf_batch_autotune(dsaxbf_batch *b, int reps, float *ms) {
   dsaxbf_clear_error();
   if (!b) return dsaxbf_fail(DSAXBF_EINVAL, "NULL batch");
+  if (reps < 1) return dsaxbf_fail(DSAXBF_EINVAL, "reps must be >= 1");
   if (hipSetDevice(b->device) != hipSuccess) return dsaxbf_fail(DSAXBF_EHIP, "hipSetDevice");
   if (ms)
     for (int i = 0; i < b->n * DSAXBF_NVARIANT; i++) ms[i] = -1.f;
-  int *best = malloc(sizeof(int) * b->n);
-  if (!best) return dsaxbf_fail(DSAXBF_ENOMEM, "host allocation failed");
+  /* candidate assignments: [0] the variants the plans had, [1] each shape's
+     fastest alone, [2] the shared schedule everywhere */
+  int *cand = malloc(sizeof(int) * 3 * b->n);
+  if (!cand) return dsaxbf_fail(DSAXBF_ENOMEM, "host allocation failed");
   int rc = DSAXBF_OK;
   for (int i = 0; i < b->n && rc == DSAXBF_OK; i++) {
-    best[i] = -1;
+    cand[i] = dsaxbf_get_variant(b->plans[i]);
+    cand[2 * b->n + i] = DSAXBF_VARIANT_SHARED_256;
+    int *best = &cand[b->n + i];
+    *best = -1;
     for (int k = 0; k < i; k++)
       if (same_shape(&b->dims[k], &b->dims[i])) {
-        best[i] = best[k];
+        *best = cand[b->n + k];
         break;
       }
-    if (best[i] >= 0) continue;
+    if (*best >= 0) continue;
     rc = dsaxbf_time_variants(b->plans[i], b->bufs[i].d_in, b->bufs[i].d_scale, b->bufs[i].d_out, reps,
-                              ms ? ms + (size_t)i * DSAXBF_NVARIANT : NULL, &best[i]);
+                              ms ? ms + (size_t)i * DSAXBF_NVARIANT : NULL, best);
   }
-  for (int i = 0; i < b->n && rc == DSAXBF_OK; i++) rc = dsaxbf_set_variant(b->plans[i], best[i]);
-  free(best);
-  if (rc == DSAXBF_OK) rc = build(b);
+  /* the whole graph decides between the candidates */
+  int pick = 0;
+  float best_ms = -1.f;
+  for (int c = 0; c < 3 && rc == DSAXBF_OK; c++) {
+    float t = 0.f;
+    rc = apply(b, cand + c * b->n);
+    if (rc == DSAXBF_OK) rc = time_graph(b, reps, &t);
+    if (rc == DSAXBF_OK && (best_ms < 0.f || t < best_ms)) {
+      best_ms = t;
+      pick = c;
+    }
+  }
+  if (rc == DSAXBF_OK) rc = apply(b, cand + pick * b->n);
+  free(cand);
   return rc;
 }

@@ -48,6 +48,7 @@ void dsaxbf_config_default(dsaxbf_config *cfg, int nant) {
   cfg->stuffants = DSAXBF_STUFF_NONE;
   cfg->input_order = DSAXBF_ORDER_CAPTURE;
   cfg->device = 0;
+  cfg->output_layout = DSAXBF_LAYOUT_TIME_MAJOR;
 }
 
 size_t dsaxbf_input_bytes(const dsaxbf_config *cfg) {
@@ -84,6 +85,8 @@ int dsaxbf_check_config(const dsaxbf_config *c) {
     return dsaxbf_fail(DSAXBF_EINVAL, "stuffants=%d unknown", c->stuffants);
   if (c->input_order != 0 && c->input_order != 1)
     return dsaxbf_fail(DSAXBF_EINVAL, "input_order=%d unknown", c->input_order);
+  if (c->output_layout != DSAXBF_LAYOUT_TIME_MAJOR && c->output_layout != DSAXBF_LAYOUT_BEAM_MAJOR)
+    return dsaxbf_fail(DSAXBF_EINVAL, "output_layout=%d unknown", c->output_layout);
   return DSAXBF_OK;
 }
 

@@ -58,6 +58,7 @@ static void fill_dims(const dsaxbf_config *c, bfk_dims *d) {
   d->prec = c->precision;
   d->order = c->input_order;
   d->stuff = c->stuffants;
+  d->layout = c->output_layout;
   /* the autotuner's usual picks on MI355X: shared 256-beam tiles from 129
    * beams (full block: 64 antennas, 256 beams; 256 antennas, 1024 beams),
    * 128- or 64-beam classic tiles below; FP6 runs shared tiles only */
@@ -262,8 +263,10 @@ int dsaxbf_get_weights(dsaxbf_plan *p, double *w) {
   return DSAXBF_OK;
 }
 
-int dsaxbf_execute(dsaxbf_plan *p, const uint8_t *d_in, const float *d_scale, uint8_t *d_out,
-                   float *d_power, float *d_beam_power, void *stream) {
+/* One block on device buffers with the per-unit beam-power partial sums in
+ * bpart (kernel launches only). */
+static int execute_with(dsaxbf_plan *p, float *bpart, const uint8_t *d_in, const float *d_scale,
+                        uint8_t *d_out, float *d_power, float *d_beam_power, void *stream) {
   if (!p || !d_in || !d_scale || !d_out) return dsaxbf_fail(DSAXBF_EINVAL, "NULL argument");
   const bfk_dims *d = &p->dims;
   if (!p->have_weights && d->stuff != DSAXBF_STUFF_ANTS)
@@ -271,18 +274,24 @@ int dsaxbf_execute(dsaxbf_plan *p, const uint8_t *d_in, const float *d_scale, ui
   void *s = stream ? stream : (void *)p->stream;
   int rc;
   if (d->stuff != DSAXBF_STUFF_ANTS) {
-    rc = bfk_beamform(d, d_in, p->d_wfrag, p->d_wscale, d_scale, d_out, d_power, p->d_bpart, s);
+    rc = bfk_beamform(d, d_in, p->d_wfrag, p->d_wscale, d_scale, d_out, d_power, bpart, s);
     if (rc) return dsaxbf_fail(rc, "%s", bfk_last_error());
   }
   if (d->stuff != DSAXBF_STUFF_NONE) {
-    rc = bfk_stuff(d, d_in, d_scale, d_out, d_power, p->d_bpart, s);
+    rc = bfk_stuff(d, d_in, d_scale, d_out, d_power, bpart, s);
     if (rc) return dsaxbf_fail(rc, "%s", bfk_last_error());
   }
   if (d_beam_power) {
-    rc = bfk_beam_power(d, p->d_bpart, d_beam_power, s);
+    rc = bfk_beam_power(d, bpart, d_beam_power, s);
     if (rc) return dsaxbf_fail(rc, "%s", bfk_last_error());
   }
   return DSAXBF_OK;
+}
+
+int dsaxbf_execute(dsaxbf_plan *p, const uint8_t *d_in, const float *d_scale, uint8_t *d_out,
+                   float *d_power, float *d_beam_power, void *stream) {
+  if (!p) return dsaxbf_fail(DSAXBF_EINVAL, "NULL plan");
+  return execute_with(p, p->d_bpart, d_in, d_scale, d_out, d_power, d_beam_power, stream);
 }
 
 int dsaxbf_beam_power(dsaxbf_plan *p, float *d_beam_power, void *stream) {

@@ -21,6 +21,9 @@
  *   -P prec   fp16 | fp8split | fp6split | fp6split3 (fp16)   -W mode  group | fine (group)
  *   -A mode   theta | sin (theta)              -r       input is dsaX_split -r order
  *   -N n      stop after n blocks              -w       re-read weights every block
+ *   -B        beam-major output [beam][t][ch]: each 64-beam slice is the buffer
+ *             dsaX_dbnic's transmit thread builds by its transpose
+ *             (dsaX_dbnic.c:111-116), so the corner turn needs none
  */
 #define _GNU_SOURCE
 #include <sched.h>
@@ -49,7 +52,7 @@ static void usage(void) {
           " -h print usage\n"
           " -I infile (- = stdin)   -O outfile (- = stdout)   -n nant [63]\n"
           " -T tavg [8]  -P fp16|fp8split|fp6split|fp6split3  -W group|fine  -A theta|sin  -r  -N nblocks\n"
-          " -w\n");
+          " -w  -B (beam-major output)\n");
 }
 
 /* Read exactly n bytes unless EOF; returns bytes read. */
@@ -70,12 +73,13 @@ int main(int argc, char *argv[]) {
   int core = -1, debug = 0, stuffants = 0, test_pattern = 0, agc_on = 1;
   int nant = 63, tavg = 8, prec = DSAXBF_PREC_FP16, wmode = DSAXBF_WEIGHTS_GROUP;
   int amode = DSAXBF_ANGLE_THETA, order = DSAXBF_ORDER_CAPTURE, reload = 0;
+  int layout = DSAXBF_LAYOUT_TIME_MAJOR;
   long max_blocks = -1;
   unsigned in_key = DSAXBF_REORDER_BLOCK_KEY2, out_key = DSAXBF_BF_BLOCK_KEY;
   float fch1 = 1530.0f;
   const char *fnam = "nofile", *flagants = "nofile", *infile = NULL, *outfile = NULL;
   int arg;
-  while ((arg = getopt(argc, argv, "c:f:i:o:z:a:tsqdghI:O:n:T:P:W:A:rN:w")) != -1) {
+  while ((arg = getopt(argc, argv, "c:f:i:o:z:a:tsqdghI:O:n:T:P:W:A:rN:wB")) != -1) {
     switch (arg) {
       case 'c': core = atoi(optarg); break;
       case 'i':
@@ -119,6 +123,7 @@ int main(int argc, char *argv[]) {
       case 'r': order = DSAXBF_ORDER_SPLIT_R; break;
       case 'N': max_blocks = atol(optarg); break;
       case 'w': reload = 1; break;
+      case 'B': layout = DSAXBF_LAYOUT_BEAM_MAJOR; break;
       default: usage(); return EXIT_FAILURE;
     }
   }
@@ -161,6 +166,7 @@ int main(int argc, char *argv[]) {
   cfg.angle_mode = amode;
   cfg.stuffants = stuffants;
   cfg.input_order = order;
+  cfg.output_layout = layout;
   const size_t block_size = dsaxbf_input_bytes(&cfg), block_out = dsaxbf_output_bytes(&cfg);
   syslog(LOG_INFO, "main: have input and output block sizes %zu %zu", block_size, block_out);
 

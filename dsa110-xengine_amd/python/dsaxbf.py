@@ -34,6 +34,8 @@ STUFF_ANTS = 1
 STUFF_IB = 2
 ORDER_CAPTURE = 0
 ORDER_SPLIT_R = 1
+LAYOUT_TIME_MAJOR = 0        # [t][beam][ch], BF_BLOCK_KEY
+LAYOUT_BEAM_MAJOR = 1        # [beam][t][ch], the corner turn's transmit order
 VARIANT_CLASSIC_256 = 0
 VARIANT_RETIRED = 1          # a persistent schedule that never won: no shape runs it
 VARIANT_CLASSIC_128S = 2
@@ -56,7 +58,7 @@ class BeamformerError(RuntimeError):
 class Config(C.Structure):
     _fields_ = [(n, C.c_int) for n in (
         "nant", "nchan", "npkt", "nbeam", "tavg", "precision", "weight_mode",
-        "angle_mode", "stuffants", "input_order", "device")]
+        "angle_mode", "stuffants", "input_order", "device", "output_layout")]
 
     @property
     def nsamp(self):
@@ -130,6 +132,12 @@ def _load():
         "dsaxbf_copy_d2h": (C.c_int, [P, P, C.c_size_t, P]),
         "dsaxbf_sync": (C.c_int, [P]),
         "dsaxbf_antpol_rms": (C.c_int, [C.POINTER(Config), P, C.c_int, C.c_int, P]),
+        "dsaxbf_xengine_create": (C.c_int, [C.POINTER(P), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "dsaxbf_xengine_destroy": (None, [P]),
+        "dsaxbf_xengine_vis_bytes": (C.c_size_t, [C.c_int, C.c_int]),
+        "dsaxbf_xengine_stream": (P, [P]),
+        "dsaxbf_xengine_execute": (C.c_int, [P, P, P, P]),
+        "dsaxbf_xengine_process_block": (C.c_int, [P, P, P]),
         "dsaxbf_version": (C.c_char_p, []),
     }
     for name, (res, args) in sig.items():
@@ -152,7 +160,9 @@ EXPORTED = [
     "dsaxbf_pipe_upload", "dsaxbf_pipe_run", "dsaxbf_process_block", "dsaxbf_agc_init", "dsaxbf_agc_free",
     "dsaxbf_agc_update", "dsaxbf_requant", "dsaxbf_synth_block", "dsaxbf_synth_geometry",
     "dsaxbf_dev_alloc", "dsaxbf_dev_free", "dsaxbf_copy_h2d", "dsaxbf_copy_d2h", "dsaxbf_sync",
-    "dsaxbf_antpol_rms", "dsaxbf_version",
+    "dsaxbf_antpol_rms", "dsaxbf_version", "dsaxbf_xengine_create", "dsaxbf_xengine_destroy",
+    "dsaxbf_xengine_vis_bytes", "dsaxbf_xengine_stream", "dsaxbf_xengine_execute",
+    "dsaxbf_xengine_process_block",
 ]
 
 
@@ -167,12 +177,13 @@ def _check(rc: int):
 
 def config(nant=64, nchan=384, npkt=2048, nbeam=256, tavg=8, precision=PREC_FP16,
            weight_mode=WEIGHTS_GROUP, angle_mode=ANGLE_THETA, stuffants=STUFF_NONE,
-           input_order=ORDER_CAPTURE, device=0) -> Config:
+           input_order=ORDER_CAPTURE, device=0, output_layout=0) -> Config:
     c = Config()
     lib.dsaxbf_config_default(C.byref(c), nant)
     c.nchan, c.npkt, c.nbeam, c.tavg = nchan, npkt, nbeam, tavg
     c.precision, c.weight_mode, c.angle_mode = precision, weight_mode, angle_mode
     c.stuffants, c.input_order, c.device = stuffants, input_order, device
+    c.output_layout = output_layout
     return c
 
 
@@ -220,6 +231,16 @@ def synth_block(cfg: Config, seed: int, sigma=2.0, clip=7, tone_snr=0.0, tone_th
                                   _ptr(tc) if len(tc) else None, len(tc),
                                   _ptr(antpos), _ptr(g), _ptr(freqs), _ptr(blk)))
     return blk
+
+
+def uneven_batch_shapes(seed=2025, nblocks=24):
+    """The uneven batch workload: nblocks blocks of 64 antennas whose channel
+    counts come from {48, 96, 192, 384}, beam counts from {64, 128, 256, 512}
+    and time lengths from {1024, 2048, 4096} samples, drawn from seed.
+    Returns a list of (nchan, nbeam, nsamp)."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.choice([48, 96, 192, 384])), int(rng.choice([64, 128, 256, 512])),
+             int(rng.choice([1024, 2048, 4096]))) for _ in range(nblocks)]
 
 
 def antpol_rms(cfg: Config, block: np.ndarray, pkt: int, group: int) -> np.ndarray:
@@ -292,7 +313,9 @@ class Plan:
         assert block.size == input_bytes(cfg)
         scale = np.ascontiguousarray(scale, np.float32)
         assert scale.size == cfg.nbeam
-        out = np.empty((cfg.nout, cfg.nbeam, cfg.ngroup), np.uint8)
+        shape = ((cfg.nout, cfg.nbeam, cfg.ngroup) if cfg.output_layout == LAYOUT_TIME_MAJOR
+                 else (cfg.nbeam, cfg.nout, cfg.ngroup))
+        out = np.empty(shape, np.uint8)
         power = np.empty(out.shape, np.float32) if want_power else None
         bp = np.empty(cfg.nbeam, np.float32) if want_beam_power else None
         _check(lib.dsaxbf_process_block(self._h, _ptr(block), _ptr(scale), _ptr(out),
@@ -411,6 +434,43 @@ class Batch:
     def close(self):
         if self._h:
             lib.dsaxbf_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class XEngine:
+    """Correlator X-engine plan (dsaxbf_xengine_*): visibilities
+    [nant(nant+1)/2 baselines][nchan][2 pols] of 4-bit blocks."""
+
+    def __init__(self, nant: int, nchan: int, npkt: int, input_order=0, device=0):
+        self.nant, self.nchan, self.npkt = nant, nchan, npkt
+        self._h = C.c_void_p()
+        _check(lib.dsaxbf_xengine_create(C.byref(self._h), nant, nchan, npkt, input_order, device))
+        self.nbase = nant * (nant + 1) // 2
+        self.vis_bytes = int(lib.dsaxbf_xengine_vis_bytes(nant, nchan))
+
+    @property
+    def stream(self):
+        return lib.dsaxbf_xengine_stream(self._h)
+
+    def process_block(self, block: np.ndarray) -> np.ndarray:
+        block = np.ascontiguousarray(block, np.uint8)
+        assert block.size == self.npkt * self.nant * self.nchan * 4
+        vis = np.empty((self.nbase, self.nchan, 2, 2), np.float32)
+        _check(lib.dsaxbf_xengine_process_block(self._h, _ptr(block), _ptr(vis)))
+        return vis[..., 0].astype(np.float64) + 1j * vis[..., 1].astype(np.float64)
+
+    def execute(self, d_in, d_vis, stream=None):
+        _check(lib.dsaxbf_xengine_execute(self._h, d_in.ptr, d_vis.ptr, stream))
+
+    def close(self):
+        if self._h:
+            lib.dsaxbf_xengine_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

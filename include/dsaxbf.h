@@ -94,6 +94,15 @@ enum dsaxbf_input_order {
   DSAXBF_ORDER_CAPTURE = 0, /* [pkt][ant][chan][t][pol] (dsaX_split without -r) */
   DSAXBF_ORDER_SPLIT_R = 1  /* [pkt][chan/8][ant][8][t][pol] (dsaX_split -r)    */
 };
+enum dsaxbf_output_layout {
+  DSAXBF_LAYOUT_TIME_MAJOR = 0, /* [nsamp/tavg][nbeam][nchan/8]: BF_BLOCK_KEY
+                                   (dsaX_beamformer.cu:150-156)                */
+  DSAXBF_LAYOUT_BEAM_MAJOR = 1  /* [nbeam][nsamp/tavg][nchan/8]: beams
+                                   64s..64s+63 are the buffer dsaX_dbnic's
+                                   transmit thread s builds with its transpose
+                                   (dsaX_dbnic.c:111-116), so the corner turn
+                                   sends slices of the block as they are     */
+};
 
 /* ---- configuration --------------------------------------------------- */
 typedef struct dsaxbf_config {
@@ -108,6 +117,7 @@ typedef struct dsaxbf_config {
   int stuffants;    /* enum dsaxbf_stuff                                     */
   int input_order;  /* enum dsaxbf_input_order                               */
   int device;       /* HIP device ordinal                                     */
+  int output_layout;/* enum dsaxbf_output_layout                             */
 } dsaxbf_config;
 
 typedef struct dsaxbf_plan dsaxbf_plan;
@@ -341,6 +351,34 @@ uint8_t *dsaxbf_pipe_input(dsaxbf_pipe *pipe, int slot);  /* pinned, input_bytes
 int dsaxbf_pipe_upload(dsaxbf_pipe *pipe, int slot);
 int dsaxbf_pipe_run(dsaxbf_pipe *pipe, int slot, const float *h_scale,
                     const uint8_t **h_out, const float **h_beam_power);
+
+/* ---- correlator X-engine on the same 4-bit input (replaces dcorrelator,
+ *      dsaX_bfCorr.cu:472-569, with its reorder_input :316-356,
+ *      reorder_output :392-457 and corr_output_copy :366-385) ---------- */
+/* Visibilities of every antenna pair (i, j <= i), baseline number
+ * b = i*(i+1)/2 + j (the reference's h_idxs order, :430-437), per channel
+ * and pol (XX and YY, as the reference), summed over the whole block:
+ *   V = sum over packets and both times of conj(d_j) * d_i
+ * as vis[((b*nchan + c)*2 + pol)*2 + {0 re, 1 im}] float32, exact (integer
+ * sums; the reference accumulates in FP16).  The input is the beamformer's
+ * block (capture order, or dsaX_split -r order with input_order 1).
+ * nant 1..96 (the reference's NANTS), nchan a multiple of 8, npkt a multiple
+ * of 32. */
+typedef struct dsaxbf_xengine dsaxbf_xengine;
+int dsaxbf_xengine_create(dsaxbf_xengine **x, int nant, int nchan, int npkt,
+                          int input_order, int device);
+void dsaxbf_xengine_destroy(dsaxbf_xengine *x);
+/* Bytes of the visibility output: nant*(nant+1)/2 * nchan * 2 pols * 8. */
+size_t dsaxbf_xengine_vis_bytes(int nant, int nchan);
+void *dsaxbf_xengine_stream(dsaxbf_xengine *x);   /* the plan's hipStream_t */
+/* One block already in device memory (stream NULL = the plan's own).
+ * Writes every entry of d_vis; no synchronisation. */
+int dsaxbf_xengine_execute(dsaxbf_xengine *x, const uint8_t *d_in, float *d_vis,
+                           void *stream);
+/* Host block in, host visibilities out (copies, run, wait): the
+ * psrdada-facing call, like dsaxbf_process_block. */
+int dsaxbf_xengine_process_block(dsaxbf_xengine *x, const uint8_t *h_in,
+                                 float *h_vis);
 
 /* ---- device memory helpers (for bindings without a GPU array library) --- */
 int dsaxbf_dev_alloc(void **d_ptr, size_t bytes, int device);

@@ -187,6 +187,16 @@ uint8_t orc_requant(float x, float scale) {
   return (uint8_t)(unsigned)h;
 }
 
+/* dsaX_dbnic.c:111-116, the "yes transpose" loop of transmit(). */
+void orc_dbnic_op(const uint8_t *out, int nout, int nbeam, int nw, int nbps, int slice,
+                  uint8_t *op) {
+  for (int i = 0; i < nout; i++)
+    for (int j = 0; j < nbps; j++)
+      for (int k = 0; k < nw; k++)
+        op[(long)j * nout * nw + (long)i * nw + k] =
+            out[(long)i * nbeam * nw + (long)slice * nbps * nw + (long)j * nw + k];
+}
+
 /* badder (dsaX_beamformer.cu:78-112) accumulates every |B|^2 of the block per
  * (beam, group); calc_allbp (:482-495) sums the groups. */
 void orc_beam_power(const orc_dims *d, const double *power, double *bp) {

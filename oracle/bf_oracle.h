@@ -66,6 +66,14 @@ void orc_power(const orc_dims *d, int wmode, const uint8_t *in, const double *W,
  * conversion to int, C division, wrap mod 256 (dsaX_beamformer.cu:150-156). */
 uint8_t orc_requant(float x, float scale);
 
+/* The corner turn's transpose (dsaX_dbnic.c:111-116): transmit thread
+ * `slice` of dsaX_dbnic copies beams slice*nbps .. +nbps-1 of a BF_BLOCK_KEY
+ * block out[nout][nbeam][nw] into op[nbps][nout][nw]
+ * (op[j*nout*nw + i*nw + k] = out[i*nbeam*nw + slice*nbps*nw + j*nw + k],
+ * NSAMPS_PER_TRANSMIT = nout, NBMS = nbeam, NBEAMS_PER_BLOCK = nbps). */
+void orc_dbnic_op(const uint8_t *out, int nout, int nbeam, int nw, int nbps, int slice,
+                  uint8_t *op);
+
 /* Sum power over the block per beam (badder + calc_allbp,
  * dsaX_beamformer.cu:78-112,482-495). */
 void orc_beam_power(const orc_dims *d, const double *power, double *bp);
@@ -80,6 +88,21 @@ typedef struct orc_agc {
 } orc_agc;
 void orc_agc_init(orc_agc *s, int nbeam, int ngroup, int nints, int enabled);
 int  orc_agc_update(orc_agc *s, const float *bp_raw, float *scale);
+
+/* ---- correlator X-engine ------------------------------------------------ */
+/* Baseline number of antenna pair (i, j <= i): b = i*(i+1)/2 + j, the order
+ * of the h_idxs loop in reorder_output (dsaX_bfCorr.cu:430-437, "upper
+ * triangular order (column major) to match xGPU"). */
+long orc_baseline(int i, int j);
+/* dcorrelator (dsaX_bfCorr.cu:472-569) with reorder_input (:316-356),
+ * reorder_output (:392-457) and corr_output_copy (:366-385): for every
+ * channel c and pol q, the visibility of baseline b = (i, j <= i) is
+ *   V = sum over the block's packets and both times of conj(d_j) * d_i
+ * (the cublas products ac + bd and ad - bc of the column-major Gram matrix),
+ * written vis[((b*nchan + c)*2 + q)*2 + {0 re, 1 im}].  Exact (integer sums
+ * of 4-bit products) in int64, stored as double.  Multithreaded over
+ * channels.  nbeam and tavg of d are unused. */
+void orc_xcorr(const orc_dims *d, const uint8_t *in, double *vis);
 
 /* ---- weight encodings of the GPU path (bf_quant.c) -------------------- */
 /* Code values and round-to-nearest-even encoders, saturating at the largest

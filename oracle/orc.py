@@ -54,6 +54,10 @@ def lib():
             getattr(L, f"orc_{fmt}_decode").argtypes = [C.c_uint32]
             getattr(L, f"orc_{fmt}_encode").restype = C.c_uint32
             getattr(L, f"orc_{fmt}_encode").argtypes = [C.c_double]
+        L.orc_dbnic_op.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P]
+        L.orc_baseline.restype = C.c_long
+        L.orc_baseline.argtypes = [C.c_int, C.c_int]
+        L.orc_xcorr.argtypes = [C.POINTER(Dims), P, P]
         L.orc_quant_weights.restype = C.c_int
         L.orc_quant_weights.argtypes = [C.POINTER(Dims), C.c_int, C.c_int, P, P]
         L.cpu_beamform.restype = C.c_double
@@ -109,6 +113,36 @@ def power(d: Dims, wmode: int, block, W, stuff=0, g0=0, g1=None) -> np.ndarray:
     blk = np.ascontiguousarray(block, np.uint8)
     lib().orc_power(C.byref(d), wmode, _p(blk), _p(Wr), stuff, g0, g1, _p(out))
     return out
+
+
+def baseline(i: int, j: int) -> int:
+    """Baseline number of antenna pair (i, j <= i)."""
+    return int(lib().orc_baseline(i, j))
+
+
+def xcorr(nant: int, nchan: int, npkt: int, block, input_order=0) -> np.ndarray:
+    """Exact visibilities [nbase][nchan][2 pols] complex128 of a 4-bit block:
+    V = sum over time of conj(d_j) d_i for baseline (i, j <= i)."""
+    d = Dims(nant, nchan, npkt, 1, 2, input_order)
+    nbase = nant * (nant + 1) // 2
+    vis = np.zeros((nbase, nchan, 2, 2), np.float64)
+    blk = np.ascontiguousarray(block, np.uint8)
+    if blk.size != npkt * nant * nchan * 4:
+        raise ValueError("block size does not match the shape")
+    lib().orc_xcorr(C.byref(d), _p(blk), _p(vis))
+    return vis[..., 0] + 1j * vis[..., 1]
+
+
+def dbnic_op(out, slice_, nbps=64) -> np.ndarray:
+    """dsaX_dbnic's transmit buffer of beam slice slice_ from a BF_BLOCK_KEY
+    block out[nout][nbeam][nw]: [nbps][nout][nw] uint8."""
+    out = np.ascontiguousarray(out, np.uint8)
+    nout, nbeam, nw = out.shape
+    if (slice_ + 1) * nbps > nbeam:
+        raise ValueError("beam slice past the block")
+    op = np.empty((nbps, nout, nw), np.uint8)
+    lib().orc_dbnic_op(_p(out), nout, nbeam, nw, nbps, slice_, _p(op))
+    return op
 
 
 def decode(fmt: str, code: int) -> float:

@@ -3,16 +3,16 @@
 Every beamformed result is checked three ways:
   1. the kernel alone: the oracle's FP64 power computed with the weights the
      kernel applies (read back from the device and decoded by the library)
-     matches the GPU power within TOL_KERNEL = 1e-5 relative per element (FP32
-     accumulation is the only difference);
+     matches the GPU power within 1e-5 relative per element (FP32 accumulation
+     is the only difference; FP8-split 5e-4, see TOL_KERNEL_BY_PREC);
   2. those device weights are the oracle's own restatement of the path's
      weight encoding (binary16 / FP8 split / FP6 splits, oracle/bf_quant.c)
      applied to the exact FP64 weights;
   3. end to end against the exact FP64 weights: within 1e-3 relative for the
-     FP16 and three-part FP6 paths; the two-part FP8 and
+     FP16 and three-part FP6 paths (the path's stated bound); the two-part FP8 and
      FP6 splits do not meet 1e-3 on every element, so they are held to the
      error their own encoding predicts (the oracle with its quantised weights)
-     plus TOL_KERNEL.
+     plus the kernel tolerance.
 uint8 bytes are within +-1 LSB everywhere (distance modulo the uint8 wrap).
 Integer-valued outputs (-s, -q) are bit-exact.
 """
@@ -27,7 +27,14 @@ from conftest import GOLDEN, ROOT
 
 pytestmark = pytest.mark.gpu
 
-TOL_KERNEL = 1e-5                   # GPU vs oracle fed the device's own weights
+# GPU vs the oracle fed the device's own weights.  FP16 and FP6 products
+# (exact) sum at FP32 precision: 1e-5.  The FP8 matrix op sums the 64
+# products of an instruction in a window of about 2^16 below the largest one
+# (measured with single-MFMA probes: 256 + 63 * 2^-9 gives 256 + 56 * 2^-9,
+# 16 + 63 * 2^-9 is exact), so the products of weight components far below
+# their row's largest are truncated; measured at most 3.0e-4 (the flag-file
+# scenario's Gaussian gains), 2.6e-5 elsewhere.
+TOL_KERNEL_BY_PREC = {0: 1e-5, 1: 5e-4, 2: 1e-5, 3: 1e-5}
 TOL_E2E = {0: 1e-3, 3: 1e-3}        # vs exact FP64 weights (FP16, FP6 three-part)
 PRECS = [0, 1, 2, 3]
 
@@ -93,13 +100,13 @@ def _check(r, prec, exact=False):
     tiny = 1e-30
     # 1. the kernel against the oracle fed the weights it applies
     relk = np.abs(pw - r["ref_dev"]) / np.maximum(np.abs(r["ref_dev"]), tiny)
-    assert relk.max() <= TOL_KERNEL, f"kernel vs its own weights: max rel {relk.max():.3e}"
+    assert relk.max() <= TOL_KERNEL_BY_PREC[prec], f"kernel vs its own weights: max rel {relk.max():.3e}"
     # 3. end to end against the exact FP64 weights
     rel = np.abs(pw - ref) / np.maximum(np.abs(ref), tiny)
     if prec in TOL_E2E:
         assert rel.max() <= TOL_E2E[prec], f"max rel power error {rel.max():.3e}"
     else:
-        bound = np.abs(r["ref_enc"] - ref) / np.maximum(np.abs(ref), tiny) + TOL_KERNEL
+        bound = np.abs(r["ref_enc"] - ref) / np.maximum(np.abs(ref), tiny) + TOL_KERNEL_BY_PREC[prec]
         assert (rel <= bound).all(), f"{(rel > bound).sum()} powers beyond their encoding's error"
     du = _u8_dist(r["out"], r["ref_u8"])
     assert du.max() <= 1, f"{(du > 1).sum()} bytes off by more than 1 LSB"
@@ -150,7 +157,7 @@ def test_stuff_modes_exact(bf, orc, stuff, tavg):
         assert np.array_equal(r["power"][:, 0], r["ref"][:, 0])
         assert np.array_equal(r["out"][:, 0], r["ref_u8"][:, 0])
         relk = np.abs(r["power"][:, 1:] - r["ref_dev"][:, 1:]) / r["ref_dev"][:, 1:]
-        assert relk.max() <= TOL_KERNEL
+        assert relk.max() <= TOL_KERNEL_BY_PREC[0]
         rel = np.abs(r["power"][:, 1:] - r["ref"][:, 1:]) / r["ref"][:, 1:]
         assert rel.max() <= TOL_E2E[0]
 
@@ -380,7 +387,7 @@ def test_reference_test_bf_scenario(bf, orc, tmp_path, prec):
     live = r["ref"][:, 1:] > 0
     p1 = r["power"][:, 1:][live].astype(np.float64)
     relk = np.abs(p1 - ref_dev[:, 1:][live]) / ref_dev[:, 1:][live]
-    assert relk.max() <= TOL_KERNEL
+    assert relk.max() <= TOL_KERNEL_BY_PREC[prec]
     rel = np.abs(p1 - r["ref"][:, 1:][live]) / r["ref"][:, 1:][live]
     # Gaussian calibration gains give weights of widely different size within a
     # beam: the hardest case for FP6's two exponent bits, which its per-lane
@@ -389,7 +396,7 @@ def test_reference_test_bf_scenario(bf, orc, tmp_path, prec):
         assert rel.max() <= TOL_E2E[prec]
     else:
         enc = np.abs(ref_enc[:, 1:][live] - r["ref"][:, 1:][live]) / r["ref"][:, 1:][live]
-        assert (rel <= enc + TOL_KERNEL).all()
+        assert (rel <= enc + TOL_KERNEL_BY_PREC[prec]).all()
     assert _u8_dist(r["out"], r["ref_u8"]).max() <= 1
 
 
@@ -546,7 +553,10 @@ def test_batch_graph_matches_per_block(bf):
 
     batch = bf.Batch(plans, [b[0] for b in bufs], [b[1] for b in bufs], [b[2] for b in bufs],
                      [b[3] for b in bufs])
-    assert batch.kernels >= 3 + len(shapes)    # >= 3 kernel instantiations + beam-power sums
+    # one launch per (precision, tavg) for the 64-antenna FP16/FP8 blocks (4),
+    # one each for the 100-antenna block and the two FP6 precisions, and one
+    # for every block's beam-power sums
+    assert batch.kernels == 8
     for rep in range(3):
         clear()
         batch.launch()

@@ -254,6 +254,26 @@ def test_beam_power_sums_outputs_and_groups(orc, bf):
     assert np.allclose(orc.beam_power(d, Pc), expect_c, rtol=1e-12)
 
 
+def test_dbnic_transpose(orc):
+    """orc_dbnic_op (dsaX_dbnic.c:111-116) at the corner turn's own sizes
+    (dsaX_def.h:87-94: NSAMPS_PER_TRANSMIT 512, NBMS 256, NBEAMS_PER_BLOCK 64,
+    NW 48): on a block of distinct values, slice s is beams 64s..64s+63 moved
+    to the front, i.e. the beam-major block cut into 64-beam slices; each
+    slice is 384 packets of P_SIZE - 12 = 4096 data bytes."""
+    rng = np.random.default_rng(0)
+    out = rng.integers(0, 256, size=(512, 256, 48), dtype=np.uint8)
+    beam_major = np.ascontiguousarray(out.transpose(1, 0, 2))
+    for s in range(4):
+        op = orc.dbnic_op(out, s)
+        assert op.shape == (64, 512, 48) and op.nbytes == 384 * 4096
+        assert np.array_equal(op, beam_major[64 * s: 64 * s + 64])
+        # spot values: op[j][i][k] = out[i][64 s + j][k]
+        for j, i, k in ((0, 0, 0), (63, 511, 47), (5, 100, 7)):
+            assert op[j, i, k] == out[i, 64 * s + j, k]
+    with pytest.raises(ValueError):
+        orc.dbnic_op(out, 4)
+
+
 def test_power_steered_tone_peaks_at_its_beam(orc, bf):
     nant, nchan, npkt, nbeam = 32, 8, 64, 33
     cfg = bf.config(nant=nant, nchan=nchan, npkt=npkt, nbeam=nbeam, tavg=32)
