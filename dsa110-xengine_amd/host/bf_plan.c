@@ -28,8 +28,12 @@ struct dsaxbf_plan {
   float *d_beam_power;
   float *d_scale;
   float *d_bpart;       /* per-tile beam power sums of the last execute */
+  void *d_scratch;      /* the device's static scratch array (dsaxbf_execute_noalloc) */
   int have_weights;
 };
+
+size_t bfk_scratch_bytes(void);
+int bfk_scratch_init(int device, void **ptr);
 
 #define HIPCHK(call, what)                                                      \
   do {                                                                          \
@@ -116,6 +120,13 @@ int dsaxbf_create(dsaxbf_plan **out, const dsaxbf_config *cfg) {
   if (e != hipSuccess) {
     dsaxbf_destroy(p);
     return dsaxbf_fail(DSAXBF_ENOMEM, "device allocation failed: %s", hipGetErrorString(e));
+  }
+  /* set-up of the allocation-free entry point: the static scratch array's
+     address on this device (looked up once per device, device current) */
+  rc = bfk_scratch_init(cfg->device, &p->d_scratch);
+  if (rc) {
+    dsaxbf_destroy(p);
+    return dsaxbf_fail(rc, "%s", bfk_last_error());
   }
   *out = p;
   return DSAXBF_OK;
@@ -292,6 +303,20 @@ int dsaxbf_execute(dsaxbf_plan *p, const uint8_t *d_in, const float *d_scale, ui
                    float *d_power, float *d_beam_power, void *stream) {
   if (!p) return dsaxbf_fail(DSAXBF_EINVAL, "NULL plan");
   return execute_with(p, p->d_bpart, d_in, d_scale, d_out, d_power, d_beam_power, stream);
+}
+
+int dsaxbf_execute_noalloc(dsaxbf_plan *p, const uint8_t *d_in, const float *d_scale, uint8_t *d_out,
+                           float *d_power, float *d_beam_power, void *stream) {
+  if (!p) return dsaxbf_fail(DSAXBF_EINVAL, "NULL plan");
+  const size_t need = bfk_bpart_bytes(&p->dims), have = bfk_scratch_bytes();
+  if (need > have)
+    return dsaxbf_fail(DSAXBF_EINVAL,
+                       "block needs %zu bytes of beam-power scratch, above the %zu-byte static "
+                       "maximum (nbeam x nsamp/32 x nchan/8 x 4 bytes); use dsaxbf_execute",
+                       need, have);
+  /* every partial sum is overwritten by this launch before it is read: the
+     array needs no clearing between calls */
+  return execute_with(p, (float *)p->d_scratch, d_in, d_scale, d_out, d_power, d_beam_power, stream);
 }
 
 int dsaxbf_beam_power(dsaxbf_plan *p, float *d_beam_power, void *stream) {

@@ -103,6 +103,7 @@ def _load():
         "dsaxbf_get_weights": (C.c_int, [P, P]),
         "dsaxbf_reference_freqs": (None, [C.c_double, C.c_int, P]),
         "dsaxbf_execute": (C.c_int, [P, P, P, P, P, P, P]),
+        "dsaxbf_execute_noalloc": (C.c_int, [P, P, P, P, P, P, P]),
         "dsaxbf_process_block": (C.c_int, [P, P, P, P, P, P]),
         "dsaxbf_beam_power": (C.c_int, [P, P, P]),
         "dsaxbf_set_variant": (C.c_int, [P, C.c_int]),
@@ -154,7 +155,7 @@ EXPORTED = [
     "dsaxbf_config_default", "dsaxbf_input_bytes", "dsaxbf_output_bytes", "dsaxbf_last_error",
     "dsaxbf_create", "dsaxbf_destroy", "dsaxbf_get_config", "dsaxbf_get_stream",
     "dsaxbf_read_weights_file", "dsaxbf_set_weights", "dsaxbf_get_weights", "dsaxbf_reference_freqs",
-    "dsaxbf_execute", "dsaxbf_beam_power", "dsaxbf_set_variant", "dsaxbf_get_variant",
+    "dsaxbf_execute", "dsaxbf_execute_noalloc", "dsaxbf_beam_power", "dsaxbf_set_variant", "dsaxbf_get_variant",
     "dsaxbf_autotune", "dsaxbf_batch_create", "dsaxbf_batch_autotune", "dsaxbf_batch_launch",
     "dsaxbf_batch_kernels", "dsaxbf_batch_destroy", "dsaxbf_pipe_create", "dsaxbf_pipe_destroy", "dsaxbf_pipe_input",
     "dsaxbf_pipe_upload", "dsaxbf_pipe_run", "dsaxbf_process_block", "dsaxbf_agc_init", "dsaxbf_agc_free",
@@ -327,6 +328,13 @@ class Plan:
             return None if x is None else (x.ptr if isinstance(x, DeviceBuffer) else C.c_void_p(x))
         _check(lib.dsaxbf_execute(self._h, p(d_in), p(d_scale), p(d_out), p(d_power),
                                   p(d_beam_power), stream))
+
+    def execute_noalloc(self, d_in, d_scale, d_out, d_power=None, d_beam_power=None, stream=None):
+        """dsaxbf_execute_noalloc: the partial sums in the library's static scratch."""
+        def p(x):
+            return None if x is None else (x.ptr if isinstance(x, DeviceBuffer) else C.c_void_p(x))
+        _check(lib.dsaxbf_execute_noalloc(self._h, p(d_in), p(d_scale), p(d_out), p(d_power),
+                                          p(d_beam_power), stream))
 
     def beam_power(self, d_beam_power, stream=None):
         """Per-beam power sum of the last execute (device buffer of nbeam floats)."""

@@ -201,6 +201,24 @@ int dsaxbf_execute(dsaxbf_plan *plan, const uint8_t *d_in, const float *d_scale,
                    uint8_t *d_out, float *d_power, float *d_beam_power,
                    void *stream);
 
+/* dsaxbf_execute that allocates nothing and uses no plan-owned scratch.  No
+ * counterpart in the reference: for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The fused kernel's per-32-sample beam-power partial sums go to the
+ * library's static device array (DSAXBF_NOALLOC_SCRATCH_BYTES per device,
+ * held until the process ends) instead of the plan's buffer; every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU race (run
+ * them in order on one stream).  Same arguments, results and asynchrony as
+ * dsaxbf_execute.  A block whose partial sums exceed the array
+ * (nbeam * nsamp/32 * nchan/8 * 4 bytes; the full 256-beam DSA block needs
+ * 6.3 MB) is DSAXBF_EINVAL.  Ask for d_beam_power in the call itself:
+ * dsaxbf_beam_power reads the plan's own buffer, which this call leaves
+ * alone. */
+#define DSAXBF_NOALLOC_SCRATCH_BYTES (8u << 20)
+int dsaxbf_execute_noalloc(dsaxbf_plan *plan, const uint8_t *d_in, const float *d_scale,
+                           uint8_t *d_out, float *d_power, float *d_beam_power,
+                           void *stream);
+
 /* Per-beam power sum of the block most recently run by dsaxbf_execute on
  * this plan: d_beam_power[b] = sum over outputs and groups of the power
  * (badder + calc_allbp, dsaX_beamformer.cu:78-112,482-495), from partial
