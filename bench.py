@@ -325,28 +325,35 @@ def stream_bench(dsaxbf, ev, nblocks=8, seed=77):
 def xengine_bench(dsaxbf, ev, reps=20, seed=91):
     """The correlator X-engine on a full block of the reference correlator's
     shape: 96 antennas x 384 channels x 4096 samples (2048 packets, 302 MB
-    packed), all 4656 baselines x 2 pols, visibilities exact.  Kernel time
-    from HIP events on the plan's stream over reps launches, three distinct
-    blocks in rotation (larger together than the last-level cache)."""
+    packed), all 4656 baselines x 2 pols, visibilities exact.  Time per block
+    (the turn, the correlation and the slice sum) from HIP events on the
+    plan's stream over reps executes, three distinct blocks in rotation
+    (larger together than the last-level cache), for several time-slice
+    counts; the library's default is the headline."""
     nant, nchan, npkt = 96, 384, 2048
     cfg = dsaxbf.config(nant=nant, nchan=nchan, npkt=npkt, nbeam=8)
     blocks = [dsaxbf.synth_block(cfg, seed + k) for k in range(3)]
     x = dsaxbf.XEngine(nant, nchan, npkt)
+    default_slices = x.slices
     d_in = [dsaxbf.DeviceBuffer(b.nbytes) for b in blocks]
     for buf, b in zip(d_in, blocks):
         buf.upload(b)
     d_vis = dsaxbf.DeviceBuffer(x.vis_bytes)
     stream = x.stream
-    for i in range(3):
-        x.execute(d_in[i], d_vis, stream)
-    ev.sync()
-    marks = ev.create(2 * reps)
-    for i in range(reps):
-        ev.record(marks[2 * i], stream)
-        x.execute(d_in[i % 3], d_vis, stream)
-        ev.record(marks[2 * i + 1], stream)
-    ms = [ev.ms(marks[2 * i], marks[2 * i + 1]) for i in range(reps)]
-    kern_ms = float(np.mean(ms))
+    sweep = {}
+    for ns in sorted({1, 2, default_slices, 4, 8}):
+        x.slices = ns
+        for i in range(3):
+            x.execute(d_in[i], d_vis, stream)
+        ev.sync()
+        marks = ev.create(2 * reps)
+        for i in range(reps):
+            ev.record(marks[2 * i], stream)
+            x.execute(d_in[i % 3], d_vis, stream)
+            ev.record(marks[2 * i + 1], stream)
+        ms = [ev.ms(marks[2 * i], marks[2 * i + 1]) for i in range(reps)]
+        sweep[ns] = (float(np.mean(ms)), float(np.min(ms)))
+    kern_ms, kern_min = sweep[default_slices]
     nin = blocks[0].nbytes
     alg_bytes = nin + x.vis_bytes
     nbase = nant * (nant + 1) // 2
@@ -359,14 +366,104 @@ def xengine_bench(dsaxbf, ev, reps=20, seed=91):
         b.free()
     return {"workload": "96 ant x 2 pol x 384 ch x 4096 samples (302 MB packed, capture order) -> "
                         "4656 baselines x 384 ch x 2 pols, exact integer visibilities",
-            "kernel": "xc_kernel<3>", "kernel_ms": kern_ms, "kernel_ms_min": float(np.min(ms)),
+            "kernels": "xc_turn_kernel + xc_kernel<3> + xc_reduce_kernel", "slices": default_slices,
+            "ms_per_block": kern_ms, "ms_per_block_min": kern_min,
+            "ms_by_slices": {str(k): v[0] for k, v in sweep.items()},
             "samples_per_s": nant * 2 * nchan * 2 * npkt / (kern_ms / 1e3),
             "roofline": {"bound": "hbm", "achieved": alg_bytes / (kern_ms / 1e3) / 1e9, "peak": 8000.0,
                          "unit": "GB/s", "frac": alg_bytes / (kern_ms / 1e3) / 1e9 / 8000.0,
-                         "bytes_per_launch": alg_bytes},
+                         "bytes_per_launch": alg_bytes,
+                         "note": "algorithmic bytes (block in + visibilities out) over the whole "
+                                 "three-launch sequence; the turn adds a block write and read"},
             "mfma_fp6_executed_frac": flop_exec / (kern_ms / 1e3) / 1e12 / PEAK_TFLOPS["fp6split"],
             "algorithmic_tflops": flop_alg / (kern_ms / 1e3) / 1e12,
             "realtime_fraction": kern_ms / 134.217728}
+
+
+def flagger_bench(dsaxbf, ev, reps=6, seed=5, cpu_sample_beams=2):
+    """The search-side RFI flagger on the reference's block: 64 beams x 16384
+    times x 1024 channels uint8 (1 GiB), three synthetic blocks drawn by the
+    library's noise generator (N(48, 6) bytes) in rotation.  Timed with HIP
+    events on the plan's stream: the statistics pass (read 1 GiB), scaley
+    fused with the statistics (read + write 1 GiB), flag fused with make_ts
+    (read 1 GiB, write the masked columns), and the whole block step with its
+    host work (wall clock).  The oracle's step on cpu_sample_beams beams,
+    scaled to the block, is the CPU reference time."""
+    import dsaxfl
+    nb, nt, nc = 64, 16384, 1024
+    n = nb * nt * nc
+    nbc = nb * nc
+    cfg = dsaxfl.config(nb, nt, nc, tsflag=1, pwr=1, seed=seed)
+    plan = dsaxfl.Plan(cfg)
+    s = plan.stream
+    blocks = [dsaxbf.DeviceBuffer(n) for _ in range(3)]
+    for k, b in enumerate(blocks):
+        dsaxfl.genrand(b, n, seed + 1 + k, s)
+    outs = [dsaxbf.DeviceBuffer(4 * nbc) for _ in range(4)]
+    plan.calibrate(blocks)
+    d_s0, d_v0 = dsaxbf.DeviceBuffer(4 * nbc), dsaxbf.DeviceBuffer(4 * nbc)
+    s0, v0 = plan.reference()
+    d_s0.upload(s0)
+    d_v0.upload(v0)
+    rng = np.random.default_rng(seed)
+    mask = (rng.random((nb, nc)) < 0.05).astype(np.uint8)
+    d_mask, d_bp, d_ts = dsaxbf.DeviceBuffer(nbc), dsaxbf.DeviceBuffer(4 * nb), dsaxbf.DeviceBuffer(4 * nb * nt)
+    d_mask.upload(mask)
+    d_bp.upload(np.ones(nb, np.float32))
+    ev.sync()
+
+    def timed(fn):
+        fn(0)
+        ev.sync()
+        marks = ev.create(2 * reps)
+        for i in range(reps):
+            ev.record(marks[2 * i], s)
+            fn(i % 3)
+            ev.record(marks[2 * i + 1], s)
+        return float(np.mean([ev.ms(marks[2 * i], marks[2 * i + 1]) for i in range(reps)]))
+
+    t_stats = timed(lambda i: dsaxfl.stats(blocks[i], nb, nt, nc, outs[0], outs[1], outs[2], outs[3], stream=s))
+    t_scaley = timed(lambda i: dsaxfl.scaley(blocks[i], nb, nt, nc, d_s0, d_v0, outs[0], outs[1], None, outs[3],
+                                             stream=s))
+    t_flag = timed(lambda i: dsaxfl.flag_mask(blocks[i], nb, nt, nc, d_mask, d_bp, d_ts, stream=s))
+    plan.set_total_power(0.0)
+    plan.step(blocks[0])                       # the first block after start-up is noise-replaced
+    walls, infos = [], []
+    for i in range(reps):
+        t0 = time.perf_counter()
+        infos.append(plan.step(blocks[i % 3]))
+        walls.append(time.perf_counter() - t0)
+    step_s = float(np.median(walls))
+    # the oracle's step on a sample of beams, scaled to the block
+    import orc
+    x = np.random.default_rng(seed).normal(48, 6, (cpu_sample_beams, nt, nc))
+    x = np.clip(np.rint(x), 0, 255).astype(np.uint8)
+    rep = np.full(nt * nc, 48, np.uint8)
+    t0 = time.perf_counter()
+    orc.fl_step(x, s0[:cpu_sample_beams], v0[:cpu_sample_beams], rep, 1.0, 1e9, 5.0, 10.0, pwr=1, tsflag=1,
+                exact_var=False)
+    cpu_s = (time.perf_counter() - t0) * nb / cpu_sample_beams
+    plan.close()
+    for b in blocks + outs + [d_s0, d_v0, d_mask, d_bp, d_ts]:
+        b.free()
+    flagged_bytes = int(mask.sum()) * nt
+    hbm = {"stats": n + 4 * 4 * nbc, "scaley_stats": 2 * n + 3 * 4 * nbc,
+           "flag_make_ts": n + flagged_bytes + 4 * nb * nt}
+    ms = {"stats": t_stats, "scaley_stats": t_scaley, "flag_make_ts": t_flag}
+    return {"workload": "64 beams x 16384 times x 1024 channels uint8 (1 GiB), synthetic N(48,6) noise, "
+                        "5% of (beam, channel) columns masked for the flag pass",
+            "kernel_ms": ms,
+            "roofline": {k: {"bound": "hbm", "achieved": hbm[k] / (ms[k] / 1e3) / 1e9, "peak": 8000.0,
+                             "unit": "GB/s", "frac": hbm[k] / (ms[k] / 1e3) / 1e9 / 8000.0,
+                             "bytes_per_launch": hbm[k]} for k in ms},
+            "step_wall_s": step_s, "block_s": nt * 1e-3, "realtime_margin": nt * 1e-3 / step_s,
+            "step_info": [{"tpwr": i.tpwr, "replaced": i.replaced, "n_mask": i.n_mask, "n_tsmask": i.n_tsmask}
+                          for i in infos[:3]],
+            "reference_kernel_passes": "scaley r+w, calc_spectrum x2, calc_varspec, calc_ppspec, flag, make_ts: "
+                                       "6 reads + 1 write of the block (gpu_flagger.cu:1297-1384)",
+            "cpu_oracle_step_s_per_block": cpu_s,
+            "cpu_sample": f"oracle step (reference float order, bubble-sort medians) on {cpu_sample_beams} "
+                          f"beams, scaled x{nb // cpu_sample_beams}, 1 thread"}
 
 
 def batch_bench(dsaxbf, ev, seed=2025, nblocks=24, reps=5):
@@ -462,6 +559,8 @@ def main():
                     help="skip the streamed scaled-array measurement (256 antennas, 1024 beams)")
     ap.add_argument("--no-xengine", action="store_true",
                     help="skip the correlator X-engine measurement (96 antennas, full block)")
+    ap.add_argument("--no-flagger", action="store_true",
+                    help="skip the RFI flagger measurement (64 beams x 16384 x 1024 block)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy, three times: the "
                          "known-size kernel that scales the HBM byte counters")
@@ -530,6 +629,14 @@ def main():
             xeng = {"error": f"{type(exc).__name__}: {exc}"}
         print(f"# xengine: {xeng}", file=sys.stderr, flush=True)
 
+    flag = None
+    if not args.no_flagger:
+        try:
+            flag = flagger_bench(dsaxbf, ev)
+        except Exception as exc:             # reported, never fatal to the headline line
+            flag = {"error": f"{type(exc).__name__}: {exc}"}
+        print(f"# flagger: {flag}", file=sys.stderr, flush=True)
+
     batch = None
     if not args.no_batch:
         try:
@@ -581,6 +688,7 @@ def main():
         "scaled_array_stream": stream,
         "uneven_batch_graph": batch,
         "correlator_xengine": xeng,
+        "rfi_flagger": flag,
     }
     print(json.dumps(line), flush=True)
 

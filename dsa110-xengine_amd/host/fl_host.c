@@ -1,0 +1,173 @@
+/*
+ * fl_host.c -- the flagger's CPU functions (gpu_flagger.cu:530-790): medians,
+ * median absolute deviations and the channel / time masks derived from the
+ * GPU statistics.
+ *
+ * The reference sorts a copy by bubble sort for every median (medval,
+ * :532-552), 64 x 3 x 768-element sorts plus 49k 21-element sorts per block;
+ * here a median is a selection (Hoare partitioning, insertion sort for short
+ * runs), which returns the same element of the sorted order.  Every float and
+ * double operation of the mask tests is kept as the reference writes it.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "dsaxfl.h"
+
+#define FL_Z DSAXFL_ZERO_CHANNELS
+#define FL_FILT 21
+
+/* k-th smallest of a[0..n-1] (reorders a). */
+static float select_k(float *a, int n, int k) {
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 16) {
+    const int mid = lo + (hi - lo) / 2;
+    /* median of three as pivot */
+    float x = a[lo], y = a[mid], z = a[hi];
+    float p = x < y ? (y < z ? y : (x < z ? z : x)) : (x < z ? x : (y < z ? z : y));
+    int i = lo, j = hi;
+    while (i <= j) {
+      while (a[i] < p) i++;
+      while (a[j] > p) j--;
+      if (i <= j) {
+        float t = a[i];
+        a[i] = a[j];
+        a[j] = t;
+        i++;
+        j--;
+      }
+    }
+    if (k <= j) hi = j;
+    else if (k >= i) lo = i;
+    else return a[k];
+  }
+  for (int i = lo + 1; i <= hi; i++) {
+    float v = a[i];
+    int j = i - 1;
+    while (j >= lo && a[j] > v) {
+      a[j + 1] = a[j];
+      j--;
+    }
+    a[j + 1] = v;
+  }
+  return a[k];
+}
+
+static float median_of(const float *a, int n, float *tmp) {
+  memcpy(tmp, a, sizeof(float) * n);
+  return select_k(tmp, n, (n + 1) / 2 - 1);
+}
+
+float dsaxfl_medval(const float *a, int n) {
+  if (n < 1) return 0.f;
+  float *tmp = malloc(sizeof(float) * n);
+  if (!tmp) return NAN;
+  float r = median_of(a, n, tmp);
+  free(tmp);
+  return r;
+}
+
+void dsaxfl_channflag(const float *spec, int nbeam, int nchan, float thr, const int *perm, int nperm,
+                      int *mask) {
+  const int nin = nchan - 2 * FL_Z;
+  if (nin < FL_FILT) return;
+  float *cs = malloc(sizeof(float) * nin), *tmp = malloc(sizeof(float) * nin);
+  if (!cs || !tmp) {
+    free(cs);
+    free(tmp);
+    return;
+  }
+  for (int b = 0; b < nbeam; b++) {
+    const float *row = spec + (size_t)b * nchan;
+    for (int j = FL_Z; j < nchan - FL_Z; j++) {
+      const int w0 = nchan - FL_Z - j >= FL_FILT ? j : nchan - FL_Z - FL_FILT;
+      cs[j - FL_Z] = row[j] - median_of(row + w0, FL_FILT, tmp);
+    }
+    const float med = median_of(cs, nin, tmp);
+    for (int j = 0; j < nin; j++) tmp[j] = fabsf(cs[j] - med);
+    const float mad = select_k(tmp, nin, (nin + 1) / 2 - 1);
+    const float lim = thr * mad;
+    int *m = mask + (size_t)b * nchan;
+    for (int j = FL_Z; j < nchan - FL_Z; j++) {
+      const float v = cs[j - FL_Z];
+      if (v > lim || v < -lim) m[j] = 1;
+    }
+    for (int k = 0; k < nperm; k++)
+      if (perm[k] >= FL_Z && perm[k] < nchan - FL_Z) m[perm[k]] = 1;
+  }
+  free(cs);
+  free(tmp);
+}
+
+void dsaxfl_simple_channflag(const float *spec, int nbeam, int nchan, float thr, int *mask) {
+  const int nin = nchan - 2 * FL_Z;
+  if (nin < 1) return;
+  float *tmp = malloc(sizeof(float) * nin);
+  if (!tmp) return;
+  const float mythr = (float)(thr / sqrt(1. * DSAXFL_FBIN));
+  for (int b = 0; b < nbeam; b++) {
+    const float *row = spec + (size_t)b * nchan;
+    const float med = median_of(row + FL_Z, nin, tmp);
+    for (int j = 0; j < nin; j++) tmp[j] = fabsf(row[FL_Z + j] - med);
+    const float mad = select_k(tmp, nin, (nin + 1) / 2 - 1);
+    const float lim = mythr * mad;
+    for (int j = FL_Z; j < nchan - FL_Z - DSAXFL_FBIN; j++) {
+      float vv = 0.f;
+      for (int k = 0; k < DSAXFL_FBIN; k++) vv += row[j];   /* the reference adds spec[j] FBIN times */
+      vv = (float)((double)vv / (1. * DSAXFL_FBIN) - (double)med);
+      if (vv > lim) mask[(size_t)b * nchan + j] = 1;
+    }
+  }
+  free(tmp);
+}
+
+void dsaxfl_simple_tsflag(const float *ts, int nbeam, int ntime, float thr, int *mask) {
+  const int nm = ntime / 16;
+  if (nm < 1) return;
+  float *tmp = malloc(sizeof(float) * nm);
+  if (!tmp) return;
+  for (int b = 0; b < nbeam; b++) {
+    const float *row = ts + (size_t)b * ntime;
+    const float med = median_of(row, nm, tmp);
+    for (int j = 0; j < nm; j++) tmp[j] = fabsf(row[j] - med);
+    const float lim = thr * select_k(tmp, nm, (nm + 1) / 2 - 1);
+    for (int j = 0; j < ntime; j++)
+      if (row[j] - med > lim) mask[(size_t)b * ntime + j] = 1;
+  }
+  free(tmp);
+}
+
+int dsaxfl_gather_mask(const int *mask, long n, int *idx) {
+  int k = 0;
+  for (long i = 0; i < n; i++)
+    if (mask[i] == 1) idx[k++] = (int)i;
+  return k;
+}
+
+void dsaxfl_calc_bpwr(const float *spec, int nbeam, int nchan, float *bpwr) {
+  for (int b = 0; b < nbeam; b++) {
+    float s = 0.f;
+    for (int j = 0; j < nchan; j++) s += spec[(size_t)b * nchan + j];
+    bpwr[b] = (float)(((double)s / (1. * (nchan - 2 * FL_Z))) / DSAXFL_MN);
+  }
+}
+
+void dsaxfl_median_calc(float *arr, int nbeam, int nchan) {
+  float *col = malloc(sizeof(float) * nbeam);
+  if (!col) return;
+  for (int c = 0; c < nchan; c++) {
+    for (int b = 0; b < nbeam; b++) col[b] = arr[(size_t)b * nchan + c];
+    for (int i = 1; i < nbeam; i++) {        /* insertion sort: stable, as the bubble sort */
+      float v = col[i];
+      int j = i - 1;
+      while (j >= 0 && col[j] > v) {
+        col[j + 1] = col[j];
+        j--;
+      }
+      col[j + 1] = v;
+    }
+    for (int b = 0; b < nbeam; b++) arr[(size_t)b * nchan + c] = col[b];
+  }
+  free(col);
+}

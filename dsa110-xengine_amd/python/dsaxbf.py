@@ -139,6 +139,8 @@ def _load():
         "dsaxbf_xengine_stream": (P, [P]),
         "dsaxbf_xengine_execute": (C.c_int, [P, P, P, P]),
         "dsaxbf_xengine_process_block": (C.c_int, [P, P, P]),
+        "dsaxbf_xengine_set_slices": (C.c_int, [P, C.c_int]),
+        "dsaxbf_xengine_get_slices": (C.c_int, [P]),
         "dsaxbf_version": (C.c_char_p, []),
     }
     for name, (res, args) in sig.items():
@@ -163,7 +165,7 @@ EXPORTED = [
     "dsaxbf_dev_alloc", "dsaxbf_dev_free", "dsaxbf_copy_h2d", "dsaxbf_copy_d2h", "dsaxbf_sync",
     "dsaxbf_antpol_rms", "dsaxbf_version", "dsaxbf_xengine_create", "dsaxbf_xengine_destroy",
     "dsaxbf_xengine_vis_bytes", "dsaxbf_xengine_stream", "dsaxbf_xengine_execute",
-    "dsaxbf_xengine_process_block",
+    "dsaxbf_xengine_process_block", "dsaxbf_xengine_set_slices", "dsaxbf_xengine_get_slices",
 ]
 
 
@@ -475,6 +477,14 @@ class XEngine:
 
     def execute(self, d_in, d_vis, stream=None):
         _check(lib.dsaxbf_xengine_execute(self._h, d_in.ptr, d_vis.ptr, stream))
+
+    @property
+    def slices(self) -> int:
+        return lib.dsaxbf_xengine_get_slices(self._h)
+
+    @slices.setter
+    def slices(self, n: int):
+        _check(lib.dsaxbf_xengine_set_slices(self._h, n))
 
     def close(self):
         if self._h:

@@ -390,9 +390,18 @@ void dsaxbf_xengine_destroy(dsaxbf_xengine *x);
 size_t dsaxbf_xengine_vis_bytes(int nant, int nchan);
 void *dsaxbf_xengine_stream(dsaxbf_xengine *x);   /* the plan's hipStream_t */
 /* One block already in device memory (stream NULL = the plan's own).
- * Writes every entry of d_vis; no synchronisation. */
+ * Writes every entry of d_vis; no synchronisation.  Three launches: the
+ * block turned into channel-pair planes (plan-owned, the input's size), the
+ * correlation over time slices, and the sum of the slices' partial
+ * visibilities (plan-owned) when there is more than one slice. */
 int dsaxbf_xengine_execute(dsaxbf_xengine *x, const uint8_t *d_in, float *d_vis,
                            void *stream);
+/* Time slices the correlation runs in (1 .. npkt/32; 0 = the library's
+ * choice for the shape, the default).  Tuning only: results are identical
+ * (exact integer sums).  Reallocates the partial sums; not while a launch
+ * on the plan is in flight. */
+int dsaxbf_xengine_set_slices(dsaxbf_xengine *x, int nslice);
+int dsaxbf_xengine_get_slices(const dsaxbf_xengine *x);
 /* Host block in, host visibilities out (copies, run, wait): the
  * psrdada-facing call, like dsaxbf_process_block. */
 int dsaxbf_xengine_process_block(dsaxbf_xengine *x, const uint8_t *h_in,

@@ -63,6 +63,30 @@ def lib():
         L.cpu_beamform.restype = C.c_double
         L.cpu_beamform.argtypes = [C.POINTER(Dims), C.c_int, P, P, P, C.c_int, C.c_int, P, P,
                                    C.c_int]
+        i3 = [C.c_int] * 3
+        for f in ("spectrum", "maxspec", "ppspec", "make_ts"):
+            getattr(L, f"orc_fl_{f}").argtypes = [P] + i3 + [P]
+        for f in ("varspec", "varspec_exact", "scaley"):
+            getattr(L, f"orc_fl_{f}").argtypes = [P] + i3 + [P, P]
+        L.orc_fl_sumpulse.argtypes = [P, C.c_long, P]
+        L.orc_fl_flag.argtypes = [P] + i3 + [P, C.c_int, P]
+        L.orc_fl_flagts.argtypes = [P] + i3 + [P, C.c_int, P]
+        L.orc_fl_mix.restype = C.c_uint64
+        L.orc_fl_mix.argtypes = [C.c_uint64]
+        L.orc_fl_genrand.argtypes = [P, C.c_long, C.c_uint32]
+        L.orc_fl_fix_zspec.argtypes = [P, P, C.c_int, C.c_int]
+        L.orc_fl_medval.restype = C.c_float
+        L.orc_fl_medval.argtypes = [P, C.c_int]
+        L.orc_fl_simple_channflag.argtypes = [P, C.c_int, C.c_int, C.c_float, P]
+        L.orc_fl_simple_tsflag.argtypes = [P, C.c_int, C.c_int, C.c_float, P]
+        L.orc_fl_channflag.argtypes = [P, C.c_int, C.c_int, C.c_float, P, C.c_int, P]
+        L.orc_fl_gather.restype = C.c_int
+        L.orc_fl_gather.argtypes = [P, C.c_long, P]
+        L.orc_fl_calc_bpwr.argtypes = [P, C.c_int, C.c_int, P]
+        L.orc_fl_median_calc.argtypes = [P, C.c_int, C.c_int]
+        L.orc_fl_step.restype = C.c_float
+        L.orc_fl_step.argtypes = [P] + i3 + [P, P, P] + [C.c_float] * 4 + [C.c_int] * 3 + \
+            [P, C.c_int, C.c_int, P, P, P]
         _lib = L
     return _lib
 
@@ -238,3 +262,109 @@ def cpu_beamform(d: Dims, wmode: int, block, W, scale, g0=0, g1=None, nthreads=0
     secs = (native or lib()).cpu_beamform(C.byref(d), wmode, _p(np.ascontiguousarray(block, np.uint8)),
                               _p(Wf), _p(sc), g0, g1, _p(pw), _p(out), nthreads)
     return secs, pw, out
+
+
+# ---- search-side RFI flagger (oracle/fl_oracle.c) ---------------------------
+
+def _u8(a):
+    return np.ascontiguousarray(a, np.uint8)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def fl_stat(name: str, data, spec=None):
+    """One of the per-(beam, channel) statistics of a [nbeam][ntime][nchan]
+    block: spectrum, varspec (needs spec), varspec_exact (needs spec),
+    maxspec, ppspec; or make_ts (per (beam, time))."""
+    nb, nt, nc = data.shape
+    out = np.empty((nb, nt) if name == "make_ts" else (nb, nc), np.float32)
+    f = getattr(lib(), f"orc_fl_{name}")
+    if spec is None:
+        f(_p(_u8(data)), nb, nt, nc, _p(out))
+    else:
+        f(_p(_u8(data)), nb, nt, nc, _p(_f32(spec)), _p(out))
+    return out
+
+
+def fl_scaley(data, spec0, var0):
+    d = _u8(data).copy()
+    nb, nt, nc = d.shape
+    lib().orc_fl_scaley(_p(d), nb, nt, nc, _p(_f32(spec0)), _p(_f32(var0)))
+    return d
+
+
+def fl_sumpulse(data, summand):
+    d = _u8(data).copy()
+    lib().orc_fl_sumpulse(_p(d), d.size, _p(_f32(summand)))
+    return d
+
+
+def fl_flag(data, midx, bpwr, ts=False):
+    d = _u8(data).copy()
+    nb, nt, nc = d.shape
+    m = np.ascontiguousarray(midx, np.int32)
+    (lib().orc_fl_flagts if ts else lib().orc_fl_flag)(_p(d), nb, nt, nc, _p(m), len(m), _p(_f32(bpwr)))
+    return d
+
+
+def fl_genrand(n: int, seed: int):
+    r = np.empty(n, np.uint8)
+    lib().orc_fl_genrand(_p(r), n, seed)
+    return r
+
+
+def fl_fix_zspec(s0, v0, naver: int):
+    """s0, v0: [naver][...] stacks; returns the fixed copies (entry 0 holds the medians)."""
+    s, v = _f32(s0).copy(), _f32(v0).copy()
+    lib().orc_fl_fix_zspec(_p(s), _p(v), s[0].size, naver)
+    return s, v
+
+
+def fl_medval(a) -> float:
+    a = _f32(a)
+    return float(lib().orc_fl_medval(_p(a), len(a)))
+
+
+def fl_mask(kind: str, spec, thr: float, perm=(), mask=None):
+    """channflag / simple_channflag / simple_tsflag ORed into mask (int32)."""
+    s = _f32(spec)
+    nb, n = s.shape
+    m = np.zeros((nb, n), np.int32) if mask is None else np.ascontiguousarray(mask, np.int32).copy()
+    if kind == "channflag":
+        pm = np.ascontiguousarray(perm, np.int32)
+        lib().orc_fl_channflag(_p(s), nb, n, thr, _p(pm), len(pm), _p(m))
+    elif kind == "simple_channflag":
+        lib().orc_fl_simple_channflag(_p(s), nb, n, thr, _p(m))
+    else:
+        lib().orc_fl_simple_tsflag(_p(s), nb, n, thr, _p(m))
+    return m
+
+
+def fl_calc_bpwr(spec):
+    s = _f32(spec)
+    b = np.empty(s.shape[0], np.float32)
+    lib().orc_fl_calc_bpwr(_p(s), s.shape[0], s.shape[1], _p(b))
+    return b
+
+
+def fl_median_calc(arr):
+    a = _f32(arr).copy()
+    lib().orc_fl_median_calc(_p(a), a.shape[0], a.shape[1])
+    return a
+
+
+def fl_step(data, spec0, var0, repval, prev_tpwr, mod_thresh, thresh, tsthresh, pwr=1, mkrand=0,
+            tsflag=1, perm=(10,), bpwr=None, exact_var=False):
+    """The flagger's steady-state block step; returns (block, tpwr, bpwr, n_mask, n_tsmask).
+    exact_var: varspec from exact integer sums instead of the reference's float order."""
+    d = _u8(data).copy()
+    nb, nt, nc = d.shape
+    b = np.ones(nb, np.float32) if bpwr is None else _f32(bpwr).copy()
+    pm = np.ascontiguousarray(perm, np.int32)
+    nm, nts = C.c_int(), C.c_int()
+    t = lib().orc_fl_step(_p(d), nb, nt, nc, _p(_f32(spec0)), _p(_f32(var0)), _p(_u8(repval)),
+                          prev_tpwr, mod_thresh, thresh, tsthresh, pwr, mkrand, tsflag, _p(pm), len(pm),
+                          int(exact_var), _p(b), C.byref(nm), C.byref(nts))
+    return d, float(t), b, nm.value, nts.value

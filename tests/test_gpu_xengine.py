@@ -73,7 +73,27 @@ def test_xengine_packet_fixture_and_device_execute(bf, orc):
     x.close()
 
 
+@pytest.mark.parametrize("nslice", [1, 2, 3, 5, 8])
+def test_xengine_time_slices_give_the_same_visibilities(bf, orc, nslice):
+    """The correlation split into time slices whose partial visibilities are
+    summed (uneven slices too): identical to the oracle."""
+    nant, nchan, npkt = 96, 48, 256
+    cfg = bf.config(nant=nant, nchan=nchan, npkt=npkt, nbeam=8)
+    block = bf.synth_block(cfg, 500 + nslice)
+    x = bf.XEngine(nant, nchan, npkt)
+    x.slices = nslice
+    assert x.slices == nslice
+    vis = x.process_block(block)
+    x.close()
+    assert np.array_equal(vis, orc.xcorr(nant, nchan, npkt, block))
+
+
 def test_xengine_rejects_bad_shapes(bf):
     for args in ((97, 8, 32), (0, 8, 32), (32, 12, 32), (32, 8, 48)):
         with pytest.raises(bf.BeamformerError):
             bf.XEngine(*args)
+    x = bf.XEngine(32, 8, 64)
+    for n in (-1, 3):                        # 64 packets are 2 units of 32
+        with pytest.raises(bf.BeamformerError):
+            x.slices = n
+    x.close()
