@@ -8,7 +8,9 @@
 // read and written it.  Here:
 //
 //   fl_stats_kernel   one pass for every statistic, optionally scaling the
-//                     block in place first (scaley fused): a workgroup owns a
+//                     block in place first (scaley fused, as a per-channel
+//                     table of the 256 byte values built in LDS with the
+//                     reference's FP64 formula): a workgroup owns a
 //                     beam's 256-channel slab, each lane 4 channels read as
 //                     one dword per row (a wave reads 256 contiguous bytes),
 //                     the 16 waves the 32 time segments of the reference's
@@ -72,37 +74,56 @@ template <bool SCALE>
 __global__ __launch_bounds__(1024) void fl_stats_kernel(FlStats a) {
   __shared__ uint32_t lc[FL_NSEG][FL_SLAB];            // per segment: largest bin sum
   __shared__ uint32_t ls[FL_SW][FL_SLAB], lq[FL_SW][FL_SLAB], lm[FL_SW][FL_SLAB];
+  // scaley as a table: the scaled byte of every input byte x for each channel
+  // of the slab, [x][channel] so that a wave's 64 lanes (consecutive channel
+  // quads) read 64 distinct banks whatever their bytes
+  __shared__ uint8_t lut[SCALE ? 256 * FL_SLAB : 4];
   const int nslab = (a.nc + FL_SLAB - 1) / FL_SLAB;
   const int bm = blockIdx.x / nslab, slab = blockIdx.x % nslab;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int c0 = slab * FL_SLAB + lane * 4;
   const bool live = c0 < a.nc;
-  const int per = a.nt / FL_NSEG, nbin = per / FL_TBIN;
+  const int per = a.nt / FL_NSEG;
   const size_t nc = (size_t)a.nc;
-  float s0[4] = {0.f, 0.f, 0.f, 0.f};
-  double k[4] = {0., 0., 0., 0.};
-  if (SCALE && live)
-    for (int j = 0; j < 4; j++) {
-      s0[j] = a.spec0[(size_t)bm * nc + c0 + j];
-      k[j] = __ddiv_rn(FL_SIG, (double)__fsqrt_rn(a.var0[(size_t)bm * nc + c0 + j]));
+  if (SCALE) {
+    const int c = threadIdx.x & (FL_SLAB - 1), x0 = (threadIdx.x / FL_SLAB) * (256 / 4);
+    const int ch = slab * FL_SLAB + c;
+    if (ch < a.nc) {
+      const float s0 = a.spec0[(size_t)bm * nc + ch];
+      const double k = __ddiv_rn(FL_SIG, (double)__fsqrt_rn(a.var0[(size_t)bm * nc + ch]));
+      for (int x = x0; x < x0 + 256 / 4; x++) lut[x * FL_SLAB + c] = (uint8_t)fl_scale((uint32_t)x, s0, k);
     }
+    __syncthreads();
+  }
   uint32_t S[4] = {0, 0, 0, 0}, Q[4] = {0, 0, 0, 0}, M[4] = {0, 0, 0, 0};
+  // each wave walks its two segments' rows with PF loads in flight: row
+  // i + PF is loaded as row i is used (the last PF loads re-read the
+  // segment's last row).  Buffer loads over the beam's plane: 32-bit offsets,
+  // and non-live lanes read zeros past its end.
+  constexpr int PF = 16;
+  static_assert(FL_TBIN % PF == 0, "bins end on a prefetch block");
+  const size_t plane = (size_t)a.nt * nc;
+  const __amdgpu_buffer_rsrc_t rin = buffer_rsrc(a.data + (size_t)bm * plane, (uint32_t)plane);
+  const uint32_t cl = live ? (uint32_t)c0 : 0x80000000u;
   for (int h = 0; h < 2; h++) {
     const int seg = w + FL_SW * h;
-    uint32_t cm[4] = {0, 0, 0, 0};
-    for (int b = 0; b < nbin; b++) {
-      const size_t row0 = (size_t)bm * a.nt + (size_t)seg * per + (size_t)b * FL_TBIN;
-      const uint8_t *p = a.data + row0 * nc + c0;
-      uint32_t bs[4] = {0, 0, 0, 0};
-      // rows in flight per lane: 16 loads, 8 with the FP64 scaling (128 VGPRs)
-      constexpr int UNR = SCALE ? 8 : 16;
-#pragma unroll UNR
-      for (int r = 0; r < FL_TBIN; r++) {
-        uint32_t x = live ? *(const uint32_t *)(p + (size_t)r * nc) : 0u;
+    const uint32_t s0off = (uint32_t)(seg * per) * (uint32_t)a.nc;
+    uint32_t cm[4] = {0, 0, 0, 0}, bs[4] = {0, 0, 0, 0};
+    uint32_t buf[PF];
+#pragma unroll
+    for (int d = 0; d < PF; d++)
+      buf[d] = __builtin_amdgcn_raw_buffer_load_b32(rin, cl + s0off + (uint32_t)d * (uint32_t)a.nc, 0, 0);
+    for (int i0 = 0; i0 < per; i0 += PF) {
+#pragma unroll
+      for (int d = 0; d < PF; d++) {
+        uint32_t x = buf[d];
+        const uint32_t nx = (uint32_t)min(i0 + PF + d, per - 1);
+        buf[d] = __builtin_amdgcn_raw_buffer_load_b32(rin, cl + s0off + nx * (uint32_t)a.nc, 0, 0);
         if (SCALE && live) {
-          x = fl_scale(x & 0xFF, s0[0], k[0]) | fl_scale((x >> 8) & 0xFF, s0[1], k[1]) << 8 |
-              fl_scale((x >> 16) & 0xFF, s0[2], k[2]) << 16 | fl_scale(x >> 24, s0[3], k[3]) << 24;
-          *(uint32_t *)(a.wdata + (row0 + r) * nc + c0) = x;
+          const uint8_t *t = lut + lane * 4;
+          x = (uint32_t)t[(x & 0xFF) * FL_SLAB] | (uint32_t)t[((x >> 8) & 0xFF) * FL_SLAB + 1] << 8 |
+              (uint32_t)t[((x >> 16) & 0xFF) * FL_SLAB + 2] << 16 | (uint32_t)t[(x >> 24) * FL_SLAB + 3] << 24;
+          *(uint32_t *)(a.wdata + (size_t)bm * plane + (size_t)(seg * per + i0 + d) * nc + c0) = x;
         }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -112,10 +133,13 @@ __global__ __launch_bounds__(1024) void fl_stats_kernel(FlStats a) {
           M[j] = max(M[j], v);
         }
       }
+      if (((i0 + PF) & (FL_TBIN - 1)) == 0) {          // end of a 128-sample bin
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        S[j] += bs[j];
-        cm[j] = max(cm[j], bs[j]);
+        for (int j = 0; j < 4; j++) {
+          S[j] += bs[j];
+          cm[j] = max(cm[j], bs[j]);
+          bs[j] = 0;
+        }
       }
     }
 #pragma unroll
@@ -315,7 +339,8 @@ __global__ __launch_bounds__(256) void fl_fix_zspec_kernel(float *s0, float *v0,
 // ---- launchers --------------------------------------------------------------
 
 static bool fl_shape_ok(int nb, int nt, int nc, const char *who) {
-  if (nb < 1 || nt < 4096 || nt % 4096 || nc < 16 || nc % 16 || (long)nb * nt > (1L << 30) / 16) {
+  if (nb < 1 || nt < 4096 || nt % 4096 || nc < 16 || nc % 16 || (long)nb * nt > (1L << 30) / 16 ||
+      (long)nt * nc >= (1L << 31)) {
     bfk_set_error("%s: shape %d x %d x %d unsupported (ntime a multiple of 4096, nchan of 16)", who, nb,
                   nt, nc);
     return false;
