@@ -33,11 +33,13 @@ int flk_genrand(uint8_t *d_rep, size_t n, uint32_t seed, void *stream);
 int flk_noise(uint8_t *d_data, const uint8_t *d_rep, int nb, int nt, int nc, void *stream);
 int flk_fix_zspec(float *d_s0, float *d_v0, int nbc, int naver, void *stream);
 
-#define KCALL(expr)                                                   \
-  do {                                                                \
-    dsaxbf_clear_error();                                             \
-    int rc_ = (expr);                                                 \
-    if (rc_) return dsaxbf_fail(DSAXBF_EINVAL, "%s", bfk_last_error()); \
+/* launcher codes: -1 a shape or argument the kernels do not take, -3 a HIP
+   launch error */
+#define KCALL(expr)                                                                    \
+  do {                                                                                 \
+    dsaxbf_clear_error();                                                              \
+    int rc_ = (expr);                                                                  \
+    if (rc_) return dsaxbf_fail(rc_ == -3 ? DSAXBF_EHIP : DSAXBF_EINVAL, "%s", bfk_last_error()); \
   } while (0)
 
 static int nonnull(const void *p) { return p != NULL; }
@@ -68,7 +70,7 @@ int dsaxfl_make_ts(const uint8_t *d_data, int nb, int nt, int nc, float *d_ts, v
 int dsaxfl_flag(uint8_t *d_data, int nb, int nt, int nc, const int *d_midx, int n_mask, const float *d_bpwr,
                 void *stream) {
   if (!d_data || (n_mask > 0 && (!d_midx || !d_bpwr))) return dsaxbf_fail(DSAXBF_EINVAL, "NULL argument");
-  if (n_mask < 0 || n_mask > nb * nc) return dsaxbf_fail(DSAXBF_EINVAL, "n_mask=%d", n_mask);
+  if (n_mask < 0 || (long)n_mask > (long)nb * nc) return dsaxbf_fail(DSAXBF_EINVAL, "n_mask=%d", n_mask);
   KCALL(flk_flag_cols(d_data, d_midx, n_mask, d_bpwr, nb, nt, nc, stream));
   return DSAXBF_OK;
 }
@@ -176,7 +178,7 @@ int dsaxfl_create(dsaxfl_plan **out, const dsaxfl_config *cfg) {
   *out = NULL;
   const int nb = cfg->nbeam, nt = cfg->ntime, nc = cfg->nchan;
   if (nb < 1 || nt < 4096 || nt % 4096 || nc % 16 || nc <= 2 * DSAXFL_ZERO_CHANNELS + 21 ||
-      (long)nb * nt * nc > (1L << 34))
+      (long)nb * nt * nc > (1L << 34) || (long)nt * nc >= (1L << 31))
     return dsaxbf_fail(DSAXBF_EINVAL,
                        "shape %d x %d x %d: ntime a multiple of 4096, nchan a multiple of 16 above 277", nb, nt,
                        nc);
@@ -294,9 +296,10 @@ int dsaxfl_set_total_power(dsaxfl_plan *p, float prev_tpwr) {
     hipError_t e_ = (expr);                                                               \
     if (e_ != hipSuccess) return dsaxbf_fail(DSAXBF_EHIP, "%s: %s", what, hipGetErrorString(e_)); \
   } while (0)
-#define FCALL(expr)                                                        \
-  do {                                                                     \
-    if (expr) return dsaxbf_fail(DSAXBF_EHIP, "%s", bfk_last_error());     \
+#define FCALL(expr)                                                                     \
+  do {                                                                                  \
+    int rc_ = (expr);                                                                   \
+    if (rc_) return dsaxbf_fail(rc_ == -3 ? DSAXBF_EHIP : DSAXBF_EINVAL, "%s", bfk_last_error()); \
   } while (0)
 
 int dsaxfl_step(dsaxfl_plan *p, uint8_t *d_data, dsaxfl_step_info *info) {

@@ -15,7 +15,8 @@
  *
  * Shapes: ntime a multiple of 4096 (32 segments of whole 128-sample bins, as
  * the reference's 32-thread reductions and TBIN assume), nchan a multiple of
- * 16; the channel-mask functions use channels 128 .. nchan-129 (nchan > 277).
+ * 16, one beam's ntime * nchan bytes below 2 GiB; the channel-mask functions
+ * use channels 128 .. nchan-129 (nchan > 277).  A plan is not thread-safe.
  */
 #ifndef DSAXFL_H
 #define DSAXFL_H
