@@ -225,3 +225,39 @@ def test_step_matches_oracle(orc, pwr, mkrand, tsflag):
             assert nm > 0
         prev = info.tpwr
     plan.close()
+
+
+def test_full_reference_block_subset(orc):
+    """The reference's block, 64 beams x 16384 times x 1024 channels (1 GiB),
+    through the statistics pass and the fused scaley: beams 0 and 63 cut out
+    and checked against the oracle bit for bit, and over the whole block the
+    size-independent property that the time series of the scaled block sums
+    to the spectrum's total (both integer sums of the same bytes)."""
+    nb, nt, nc = 64, 16384, 1024
+    print("flagger full block", flush=True)
+    n = nb * nt * nc
+    d = DeviceBuffer(n)
+    fl.genrand(d, n, 99)                                     # N(48, 6) bytes on the device
+    outs = {k: DeviceBuffer(nb * nc * 4) for k in ("spec", "var", "max", "pp")}
+    fl.stats(d, nb, nt, nc, d_spec=outs["spec"], d_var=outs["var"], d_max=outs["max"], d_pp=outs["pp"])
+    g = {k: _get(v, (nb, nc), np.float32) for k, v in outs.items()}
+    full = _get(d, n, np.uint8).reshape(nb, nt, nc)
+    for bm in (0, nb - 1):
+        x = np.ascontiguousarray(full[bm:bm + 1])
+        spec = orc.fl_stat("spectrum", x)
+        assert np.array_equal(g["spec"][bm:bm + 1], spec), bm
+        assert np.array_equal(g["var"][bm:bm + 1], orc.fl_stat("varspec_exact", x, spec)), bm
+        assert np.array_equal(g["max"][bm:bm + 1], orc.fl_stat("maxspec", x)), bm
+        assert np.array_equal(g["pp"][bm:bm + 1], orc.fl_stat("ppspec", x)), bm
+    # scaley with the block's own spectra, then the time series of the result
+    d_s0, d_v0 = _dev(g["spec"]), _dev(np.maximum(g["var"], 1.0))
+    fl.scaley(d, nb, nt, nc, d_s0, d_v0, d_spec=outs["spec"])
+    spec2 = _get(outs["spec"], (nb, nc), np.float32).astype(np.float64)
+    d_ts = DeviceBuffer(nb * nt * 4)
+    fl.make_ts(d, nb, nt, nc, d_ts)
+    ts = _get(d_ts, (nb, nt), np.float32).astype(np.float64)
+    assert np.allclose((ts * nc).sum(axis=1), (spec2 * nt).sum(axis=1), rtol=0, atol=0.5)
+    scaled0 = _get(d, nt * nc, np.uint8).reshape(1, nt, nc)
+    assert np.array_equal(scaled0, orc.fl_scaley(full[:1], g["spec"][:1], np.maximum(g["var"], 1.0)[:1]))
+    for b in list(outs.values()) + [d, d_s0, d_v0, d_ts]:
+        b.free()
