@@ -233,6 +233,42 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *p, uin
 // sc_off = weight row * nbeam_pad + first beam are wave-uniform.
 // load(): flat loads; load_buf(): buffer loads whose offsets beyond the lane
 // part are wave-uniform (SGPRs), so no per-load VGPR address is needed.
+// Buffer sources of the weight fragments and scales for rolling prefetches
+// whose row offsets are wave-uniform: the row part of each address goes in
+// the scalar offset and the lane part is one register, where a pointer would
+// take a 64-bit address (and its arithmetic) per load.
+struct WSrc {
+  __amdgpu_buffer_rsrc_t rw, rs;
+  uint32_t vl;    // lane byte offset in a fragment record: 16 * lane (FP16), 32 * lane (FP8 / FP6)
+  uint32_t vn4;   // 4 * beam column n in a scale row
+};
+__device__ __forceinline__ i8v rb_ld32(const WSrc &src, uint32_t vo, uint32_t so) {
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  const u4 a = __builtin_amdgcn_raw_buffer_load_b128(src.rw, vo, so, 0);
+  const u4 b = __builtin_amdgcn_raw_buffer_load_b128(src.rw, vo + 16, so, 0);
+  return (i8v){(int)a.x, (int)a.y, (int)a.z, (int)a.w, (int)b.x, (int)b.y, (int)b.z, (int)b.w};
+}
+// 24 bytes (the six operand dwords of an FP6 fragment; dwords 6, 7 undefined)
+__device__ __forceinline__ i8v rb_ld24(const WSrc &src, uint32_t vo, uint32_t so) {
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+  const u4 a = __builtin_amdgcn_raw_buffer_load_b128(src.rw, vo, so, 0);
+  const u2 b = __builtin_amdgcn_raw_buffer_load_b64(src.rw, vo + 16, so, 0);
+  typedef int i4v __attribute__((ext_vector_type(4)));
+  typedef int i2v __attribute__((ext_vector_type(2)));
+  typedef int i6v __attribute__((ext_vector_type(6)));
+  const i4v ai = __builtin_bit_cast(i4v, a);
+  const i2v bi = __builtin_bit_cast(i2v, b);
+  const i6v v = __builtin_shufflevector(ai, __builtin_shufflevector(bi, bi, 0, 1, -1, -1), 0, 1, 2, 3, 4, 5);
+  return __builtin_shufflevector(v, v, 0, 1, 2, 3, 4, 5, -1, -1);
+}
+template <int NB>
+__device__ __forceinline__ void rb_scale(float (&sc)[NB], const WSrc &src, uint32_t sc_off) {
+#pragma unroll
+  for (int bt = 0; bt < NB; bt++)
+    sc[bt] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(src.rs, src.vn4, (sc_off + bt * 32u) * 4u, 0));
+}
+
 template <int PREC, int NB = BTW, int NK = 1> struct WFrag;
 template <int NB, int NK> struct WFrag<0, NB, NK> {
   h8v w[NB][NK][8];
@@ -270,6 +306,13 @@ template <int NB, int NK> struct WFrag<0, NB, NK> {
 #pragma unroll
     for (int bt = 0; bt < NB; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
   }
+  // load_one and load_scale through WSrc (wrow_bt and sc_off wave-uniform)
+  __device__ __forceinline__ void load_one_rb(const WSrc &src, uint32_t wrow_bt, int bt, int s) {
+    static_assert(NK == 1, "single-chunk tiles");
+    w[bt][0][s] = __builtin_bit_cast(
+        h8v, __builtin_amdgcn_raw_buffer_load_b128(src.rw, src.vl, ((wrow_bt + bt) * 8u + s) * 1024u, 0));
+  }
+  __device__ __forceinline__ void load_scale_rb(const WSrc &src, uint32_t sc_off) { rb_scale<NB>(sc, src, sc_off); }
   __device__ __forceinline__ void load_buf(__amdgpu_buffer_rsrc_t rw, __amdgpu_buffer_rsrc_t rs,
                                            uint32_t wrow_bt, uint32_t sc_off, uint32_t vlane16,
                                            uint32_t vn4) {
@@ -333,6 +376,14 @@ template <int NB, int NK> struct WFrag<1, NB, NK> {
 #pragma unroll
     for (int bt = 0; bt < NB; bt++) sc[bt] = wscale[sc_off + bt * 32 + n];
   }
+  // load_one and load_scale through WSrc (wrow_bt and sc_off wave-uniform)
+  __device__ __forceinline__ void load_one_rb(const WSrc &src, uint32_t wrow_bt, int bt, int s2) {
+    static_assert(NK == 1, "single-chunk tiles");
+    const uint32_t so = ((wrow_bt + bt) * 2u + s2) * 4096u;
+    hi[bt][0][s2] = rb_ld32(src, src.vl, so);
+    lo[bt][0][s2] = rb_ld32(src, src.vl + 2048u, so);
+  }
+  __device__ __forceinline__ void load_scale_rb(const WSrc &src, uint32_t sc_off) { rb_scale<NB>(sc, src, sc_off); }
   __device__ __forceinline__ void load_buf(__amdgpu_buffer_rsrc_t rw, __amdgpu_buffer_rsrc_t rs,
                                            uint32_t wrow_bt, uint32_t sc_off, uint32_t vlane16,
                                            uint32_t vn4) {
@@ -404,6 +455,14 @@ template <int NB, int NK> struct WFrag<2, NB, NK> {
     static_assert(NK == 1, "single-chunk tiles");
     load_k(wfrag, wrow_bt, bt, 0, s2, lane);
   }
+  // load_one and load_scale through WSrc (wrow_bt and sc_off wave-uniform)
+  __device__ __forceinline__ void load_one_rb(const WSrc &src, uint32_t wrow_bt, int bt, int s2) {
+    static_assert(NK == 1, "single-chunk tiles");
+    const uint32_t so = ((wrow_bt + bt) * 2u + s2) * 4096u;
+    hi[bt][0][s2] = rb_ld32(src, src.vl, so);
+    lo[bt][0][s2] = rb_ld24(src, src.vl + 2048u, so);
+  }
+  __device__ __forceinline__ void load_scale_rb(const WSrc &src, uint32_t sc_off) { rb_scale<NB>(sc, src, sc_off); }
 };
 
 // FP6 hi + mid + lo fragments (three-part split): the hi fragment's dwords 6
@@ -439,6 +498,17 @@ template <int NB, int NK> struct WFrag<3, NB, NK> {
     static_assert(NK == 1, "single-chunk tiles");
     load_k(wfrag, wrow_bt, bt, 0, s2, lane);
   }
+  // load_one and load_scale through WSrc (wrow_bt and sc_off wave-uniform);
+  // the lo part's 4096-byte offset is past the instruction's offset field, so
+  // it goes in the scalar offset
+  __device__ __forceinline__ void load_one_rb(const WSrc &src, uint32_t wrow_bt, int bt, int s2) {
+    static_assert(NK == 1, "single-chunk tiles");
+    const uint32_t so = ((wrow_bt + bt) * 2u + s2) * 6144u;
+    hi[bt][0][s2] = rb_ld32(src, src.vl, so);
+    mid[bt][0][s2] = rb_ld32(src, src.vl + 2048u, so);
+    lo[bt][0][s2] = rb_ld24(src, src.vl, so + 4096u);
+  }
+  __device__ __forceinline__ void load_scale_rb(const WSrc &src, uint32_t sc_off) { rb_scale<NB>(sc, src, sc_off); }
 };
 
 // One 32x32x64 MFMA of the FP8 (PREC 1) or FP6 (PREC 2, 3) chains.  FP6

@@ -130,11 +130,13 @@ __device__ __forceinline__ void det_final(const float (&sd)[NP][4], float sc, fl
 // from row wnext right after its last MFMA has issued (a rolling prefetch of
 // the next (channel, pol)'s weights that needs no extra registers), pinned
 // there by scheduling barriers that VALU, SALU and LDS work may still cross.
+// The prefetches are buffer loads whose row offset (wave-uniform) is a
+// scalar: no 64-bit address arithmetic per load.
 // (For FP16 this block order measured faster than the deferred-detect chains
 // that FP8 uses: those need A_im derived once per beam tile.)
 template <bool PF>
 __device__ __forceinline__ void unit16(Acc<BTW> &A, WFrag<0, BTW, 1> &W, const uint8_t *unit, int lane,
-                                       const uint8_t *wfrag, long wnext) {
+                                       const WSrc &src, uint32_t wnext) {
   __builtin_amdgcn_s_setprio(1);
 #pragma unroll
   for (int bt = 0; bt < BTW; bt++) { A.r[bt] = (f16v){}; A.i[bt] = (f16v){}; }
@@ -152,7 +154,7 @@ __device__ __forceinline__ void unit16(Acc<BTW> &A, WFrag<0, BTW, 1> &W, const u
     if constexpr (PF) {
       __builtin_amdgcn_sched_barrier(0x386);
 #pragma unroll
-      for (int bt = 0; bt < BTW; bt++) W.load_one(wfrag, wnext, bt, s, lane);
+      for (int bt = 0; bt < BTW; bt++) W.load_one_rb(src, wnext, bt, s);
       __builtin_amdgcn_sched_barrier(0x386);
     }
   }
@@ -170,7 +172,7 @@ __device__ __forceinline__ void unit16(Acc<BTW> &A, WFrag<0, BTW, 1> &W, const u
 // barriers keep this interleave as written.
 template <int PREC, int BT, bool PF, int NP>
 __device__ __forceinline__ void chain8(f16v &cr, f16v &ci, WFrag<PREC, BTW, 1> &W, const i8v (&are)[2],
-                                       const i8v (&aim)[2], int lane, const uint8_t *wfrag, long wnext,
+                                       const i8v (&aim)[2], const WSrc &src, uint32_t wnext,
                                        const Pending &pd, float (&into)[NP]) {
   constexpr int NPART = PREC == 3 ? 3 : 2;
   float sd[NP][4];
@@ -197,7 +199,7 @@ __device__ __forceinline__ void chain8(f16v &cr, f16v &ci, WFrag<PREC, BTW, 1> &
       __builtin_amdgcn_sched_barrier(0);
       ci = mfma64<PREC>(aim[s2], *w, ci, sb);
       if (2 * m + 1 < 16) { det_elem<NP>(pd, 2 * m, sd); det_elem<NP>(pd, 2 * m + 1, sd); }
-      if (PF && pt == NPART - 1) W.load_one(wfrag, wnext, BT, s2, lane);
+      if (PF && pt == NPART - 1) W.load_one_rb(src, wnext, BT, s2);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -211,7 +213,7 @@ __device__ __forceinline__ void chain8(f16v &cr, f16v &ci, WFrag<PREC, BTW, 1> &
 // into into0, chain 1 detects chain 0 into into1, and chain 1 is left pending.
 template <int PREC, bool PF, int NP>
 __device__ __forceinline__ void unit8(Pending &pd, WFrag<PREC, BTW, 1> &W, const uint8_t *unit, int lane,
-                                      const uint8_t *wfrag, long wnext, float (&into0)[NP], float (&into1)[NP]) {
+                                      const WSrc &src, uint32_t wnext, float (&into0)[NP], float (&into1)[NP]) {
   f16v cr, ci;
   __builtin_amdgcn_s_setprio(1);
   i8v are[2], aim[2];
@@ -230,9 +232,9 @@ __device__ __forceinline__ void unit8(Pending &pd, WFrag<PREC, BTW, 1> &W, const
       f6_re_im(r0, r1, are[s2], aim[s2]);
     }
   }
-  chain8<PREC, 0, PF, NP>(cr, ci, W, are, aim, lane, wfrag, wnext, pd, into0);
+  chain8<PREC, 0, PF, NP>(cr, ci, W, are, aim, src, wnext, pd, into0);
   pd.r = cr; pd.i = ci; pd.sc = W.sc[0];
-  chain8<PREC, 1, PF, NP>(cr, ci, W, are, aim, lane, wfrag, wnext, pd, into1);
+  chain8<PREC, 1, PF, NP>(cr, ci, W, are, aim, src, wnext, pd, into1);
   __builtin_amdgcn_s_setprio(0);
   pd.r = cr; pd.i = ci; pd.sc = W.sc[1];
 }
@@ -247,7 +249,13 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
   constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;
   const TileId t = tile_id(a, L);
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: weight rows stay scalar
+  WSrc src;
+  src.rw = buffer_rsrc(a.wfrag, a.wfrag_bytes);
+  src.rs = buffer_rsrc(a.wscale, a.wscale_bytes);
+  src.vl = (uint32_t)lane * (PREC == 0 ? 16u : 32u);
+  src.vn4 = (uint32_t)(lane & 31) * 4u;
   const int n = lane & 31, h = lane >> 5;
   const int btw = (t.bset * NW + wave) * BTW;
 
@@ -313,16 +321,16 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
     unpack_unit_f16(stage, abuf, 0, tid);
     __syncthreads();
     for (int cq = 0; cq < 16; cq++) {
-      const long wn = weight_row(a, t.g, cq < 15 ? cq + 1 : cq);
+      const uint32_t wn = weight_row(a, t.g, cq < 15 ? cq + 1 : cq);
       Acc<BTW> A;
       unpack_unit_f16(stage, abuf + UNIT, 2 * cq + 1, tid);
-      unit16<false>(A, W, abuf, lane, a.wfrag, 0);
+      unit16<false>(A, W, abuf, lane, src, 0);
       detect<TAVG>(A, W.sc, part[0]);
       __syncthreads();
       if (cq < 15) unpack_unit_f16(stage, abuf, 2 * cq + 2, tid);
-      unit16<true>(A, W, abuf + UNIT, lane, a.wfrag, wn * a.nbt + btw);
+      unit16<true>(A, W, abuf + UNIT, lane, src, wn * a.nbt + btw);
       detect<TAVG>(A, W.sc, part[1]);
-      W.load_scale(a.wscale, wn * a.nbeam_pad + btw * 32, n);
+      W.load_scale_rb(src, wn * a.nbeam_pad + btw * 32);
       __syncthreads();
     }
   } else {
@@ -340,12 +348,12 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
     unpack_cq(abuf, 0);
     __syncthreads();
     for (int cq = 0; cq < 16; cq++) {
-      const long wn = weight_row(a, t.g, cq < 15 ? cq + 1 : cq);
+      const uint32_t wn = weight_row(a, t.g, cq < 15 ? cq + 1 : cq);
       if (cq < 15) unpack_cq(abuf + ((cq + 1) & 1) * AB8, cq + 1);
       const uint8_t *ab = abuf + (cq & 1) * AB8;
-      unit8<PREC, false, NP>(pd, W, ab, lane, a.wfrag, 0, part[1][1], part[0][0]);
-      unit8<PREC, true, NP>(pd, W, ab + UNIT, lane, a.wfrag, wn * a.nbt + btw, part[0][1], part[1][0]);
-      W.load_scale(a.wscale, wn * a.nbeam_pad + btw * 32, n);
+      unit8<PREC, false, NP>(pd, W, ab, lane, src, 0, part[1][1], part[0][0]);
+      unit8<PREC, true, NP>(pd, W, ab + UNIT, lane, src, wn * a.nbt + btw, part[0][1], part[1][0]);
+      W.load_scale_rb(src, wn * a.nbeam_pad + btw * 32);
       __syncthreads();
     }
     float sd[NP][4];
