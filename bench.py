@@ -357,16 +357,16 @@ def xengine_bench(dsaxbf, ev, reps=20, seed=91):
     nin = blocks[0].nbytes
     alg_bytes = nin + x.vis_bytes
     nbase = nant * (nant + 1) // 2
-    # complex MACs of the triangle (4 real MACs each), and the FP6 matrix work
-    # issued: 6 32x32 tiles x 2 (re, im) per 32 samples per (channel, pol)
+    # complex MACs of the triangle (4 real MACs each), and the int8 matrix work
+    # issued: 6 32x32 tiles x 2 (re, im) 32x32x32 ops per 16 samples per (channel, pol)
     flop_alg = 8.0 * nbase * nchan * 2 * 2 * npkt
-    flop_exec = 2.0 * 32 * 32 * 64 * 6 * 2 * (2 * npkt // 32) * nchan * 2
+    flop_exec = 2.0 * 32 * 32 * 32 * 6 * 2 * (2 * npkt // 16) * nchan * 2
     x.close()
     for b in d_in + [d_vis]:
         b.free()
     return {"workload": "96 ant x 2 pol x 384 ch x 4096 samples (302 MB packed, capture order) -> "
                         "4656 baselines x 384 ch x 2 pols, exact integer visibilities",
-            "kernels": "xc_turn_kernel + xc_kernel<3> + xc_reduce_kernel", "slices": default_slices,
+            "kernels": "xc_turn_kernel + xc_kernel<3, 2, 2> (int8) + xc_reduce_kernel", "slices": default_slices,
             "ms_per_block": kern_ms, "ms_per_block_min": kern_min,
             "ms_by_slices": {str(k): v[0] for k, v in sweep.items()},
             "samples_per_s": nant * 2 * nchan * 2 * npkt / (kern_ms / 1e3),
@@ -375,7 +375,7 @@ def xengine_bench(dsaxbf, ev, reps=20, seed=91):
                          "bytes_per_launch": alg_bytes,
                          "note": "algorithmic bytes (block in + visibilities out) over the whole "
                                  "three-launch sequence; the turn adds a block write and read"},
-            "mfma_fp6_executed_frac": flop_exec / (kern_ms / 1e3) / 1e12 / PEAK_TFLOPS["fp6split"],
+            "mfma_int8_executed_frac": flop_exec / (kern_ms / 1e3) / 1e15 / 5.0,
             "algorithmic_tflops": flop_alg / (kern_ms / 1e3) / 1e12,
             "realtime_fraction": kern_ms / 134.217728}
 

@@ -4,8 +4,8 @@
  * Replaces the reference's per-block correlator call sequence
  * (dsaX_bfCorr.cu:472-569: copy, reorder, four batched GEMMs, two reorder
  * kernels, with cudaMalloc/cublasCreate per call) by a turn of the packed
- * block into channel-pair planes, one correlation launch over time slices and
- * the sum of the slices, on device buffers the plan owns.
+ * block into (channel, pol) planes, one correlation launch over time slices
+ * and the sum of the slices, on device buffers the plan owns.
  */
 #include <hip/hip_runtime_api.h>
 #include <stdlib.h>

@@ -391,9 +391,10 @@ size_t dsaxbf_xengine_vis_bytes(int nant, int nchan);
 void *dsaxbf_xengine_stream(dsaxbf_xengine *x);   /* the plan's hipStream_t */
 /* One block already in device memory (stream NULL = the plan's own).
  * Writes every entry of d_vis; no synchronisation.  Three launches: the
- * block turned into channel-pair planes (plan-owned, the input's size), the
- * correlation over time slices, and the sum of the slices' partial
- * visibilities (plan-owned) when there is more than one slice. */
+ * block turned into (channel, pol) planes of 64-sample units (plan-owned, the
+ * input's size), the correlation over time slices, and the sum of the
+ * slices' partial visibilities (plan-owned) when there is more than one
+ * slice. */
 int dsaxbf_xengine_execute(dsaxbf_xengine *x, const uint8_t *d_in, float *d_vis,
                            void *stream);
 /* Time slices the correlation runs in (1 .. npkt/32; 0 = the library's
