@@ -262,6 +262,19 @@ __device__ __forceinline__ void xc_share(const XcArgs &a, int pq, int ub, int nu
 }
 }  // namespace
 
+// Share `part` of NS (a wave-uniform branch to the share's own code).
+template <int NB, int NS, int P, int PD>
+__device__ __forceinline__ void xc_share_of(int part, const XcArgs &a, int pq, int ub, int nunit, float *out,
+                                            int lane) {
+  if constexpr (P + 1 < NS) {
+    if (part != P) {
+      xc_share_of<NB, NS, P + 1, PD>(part, a, pq, ub, nunit, out, lane);
+      return;
+    }
+  }
+  xc_share<NB, NS, P, PD>(a, pq, ub, nunit, out, lane);
+}
+
 // NB antenna blocks of 32, NS waves per (channel, pol) plane, PD units of
 // voltages in flight.  Wave g of the grid: share g % NS of plane (g / NS) %
 // nplane of slice g / (NS nplane); the NS shares of a plane are neighbours in
@@ -283,16 +296,7 @@ __global__ __launch_bounds__(256) void xc_kernel(XcArgs a) {
   const int ntot = a.npkt / XUP;
   const int ub = (int)((long)ntot * slice / a.nslice), nunit = (int)((long)ntot * (slice + 1) / a.nslice) - ub;
   float *out = a.out + slice * a.slice_floats;
-  if constexpr (NS == 1) {
-    xc_share<NB, 1, 0, PD>(a, pq, ub, nunit, out, lane);
-  } else if constexpr (NS == 2) {
-    if (part == 0) xc_share<NB, 2, 0, PD>(a, pq, ub, nunit, out, lane);
-    else xc_share<NB, 2, 1, PD>(a, pq, ub, nunit, out, lane);
-  } else {
-    if (part == 0) xc_share<NB, 3, 0, PD>(a, pq, ub, nunit, out, lane);
-    else if (part == 1) xc_share<NB, 3, 1, PD>(a, pq, ub, nunit, out, lane);
-    else xc_share<NB, 3, 2, PD>(a, pq, ub, nunit, out, lane);
-  }
+  xc_share_of<NB, NS, 0, PD>(part, a, pq, ub, nunit, out, lane);
 }
 
 extern "C" size_t bfk_xcorr_vis_bytes(int nant, int nchan) {
@@ -357,8 +361,10 @@ extern "C" int bfk_xcorr(int nant, int nchan, int npkt, int order, const uint8_t
   a.nslice = nslice;
   a.in_bytes = in_bytes;
   a.slice_floats = (long)(bfk_xcorr_vis_bytes(nant, nchan) / sizeof(float));
-  // two waves per plane above 32 antennas (measured on the 96-antenna block:
-  // one, two and three took 0.242, 0.235 and 0.259 ms for the whole block)
+  // two waves per plane above 32 antennas (measured on the 96-antenna block,
+  // whole block: one, two, three and six waves per plane took 0.242, 0.238,
+  // 0.249 and 0.259 ms; more waves per SIMD do not pay for the duplicated
+  // loads and conversions)
   const int ns = nant <= 32 ? 1 : 2;
   const long nwave = (long)ns * 2 * nchan * nslice;
   const dim3 grid((unsigned)(((nwave + 3) / 4 + 7) / 8 * 8)), block(256);
