@@ -1,7 +1,8 @@
 """The correlator X-engine on the GPU against the exact oracle
 (oracle/xc_oracle.c, dcorrelator of dsaX_bfCorr.cu:472-569).  Visibilities
-are integer sums of 4-bit products, so the FP6 matrix-core path (products
-exact, FP32 sums below 2^24) must match the oracle bit for bit: every
+are integer sums of 4-bit products, so the int8 matrix-core path (biased
+unsigned operands, int32 sums, exact corrections) must match the oracle bit
+for bit: every
 baseline, channel and pol, both input orders, 1 to 96 antennas (the 32-antenna
 tile edges), the full 384-channel x 4096-sample block with 96 antennas on a
 channel subset, and the reference's packet.out fixture."""
