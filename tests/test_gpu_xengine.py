@@ -36,6 +36,23 @@ def test_xengine_matches_oracle_exactly(bf, orc, nant, nchan, npkt, order):
     assert np.array_equal(vis, ref)
 
 
+@pytest.mark.parametrize("order", [0, 1])
+def test_xengine_every_nibble_value_and_extremes(bf, orc, order):
+    """Uniformly random bytes, so every 4-bit value including -8 (the unsigned
+    operand 0) occurs in both components, against the oracle; and a block of
+    -8 - 8j everywhere, the largest products: Re V = 128 * nsamp, Im V = 0 for
+    every baseline, channel and pol."""
+    nant, nchan, npkt = 96, 16, 256
+    rng = np.random.default_rng(31 + order)
+    block = rng.integers(0, 256, npkt * nant * nchan * 4, dtype=np.uint8)
+    assert np.array_equal(_run(bf, nant, nchan, npkt, block, order), orc.xcorr(nant, nchan, npkt, block, order))
+    nant, nchan, npkt = 40, 8, 512
+    block = np.full(npkt * nant * nchan * 4, 0x88, np.uint8)
+    vis = _run(bf, nant, nchan, npkt, block, order)
+    assert np.array_equal(vis, orc.xcorr(nant, nchan, npkt, block, order))
+    assert (vis.real == 128 * 2 * npkt).all() and (vis.imag == 0).all()
+
+
 def test_xengine_full_block_96_antennas(bf, orc):
     """384 channels x 2048 packets x 96 antennas (the reference's NANTS):
     exact on channel group 7 (cut out of the block) and, over the whole block,
