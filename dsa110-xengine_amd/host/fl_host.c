@@ -6,7 +6,8 @@
  * The reference sorts a copy by bubble sort for every median (medval,
  * :532-552), 64 x 3 x 768-element sorts plus 49k 21-element sorts per block;
  * here a median is a selection (Hoare partitioning, insertion sort for short
- * runs), which returns the same element of the sorted order.  Every float and
+ * runs), or for channflag's 21-channel windows a sorted window that slides
+ * along the band, each returning the same element of the sorted order.  Every float and
  * double operation of the mask tests is kept as the reference writes it.
  */
 #include <math.h>
@@ -68,6 +69,44 @@ float dsaxfl_medval(const float *a, int n) {
   return r;
 }
 
+/* The 21-channel window medians channflag subtracts: window j starts at
+ * channel j, or at nchan - 128 - 21 for the windows that would pass channel
+ * nchan - 129.  Consecutive windows differ by one channel, so one sorted
+ * window slides across the band (one removal and one insertion a step, where
+ * selecting each window afresh costs a partition of all 21); its element 10
+ * is the element medval picks. */
+static void window_medians(const float *row, int nchan, float *med) {
+  float w[FL_FILT];
+  const int last = nchan - FL_Z - FL_FILT;
+  for (int i = 0; i < FL_FILT; i++) {          /* the first window, sorted */
+    const float v = row[FL_Z + i];
+    int k = i - 1;
+    while (k >= 0 && w[k] > v) {
+      w[k + 1] = w[k];
+      k--;
+    }
+    w[k + 1] = v;
+  }
+  int start = FL_Z;
+  for (int j = FL_Z; j < nchan - FL_Z; j++) {
+    const int w0 = j <= last ? j : last;
+    for (; start < w0; start++) {
+      /* drop row[start] (any element equal to it), then insert row[start + 21] */
+      const float out = row[start], in = row[start + FL_FILT];
+      int i = 0;
+      while (i < FL_FILT - 1 && w[i] != out) i++;
+      for (; i < FL_FILT - 1; i++) w[i] = w[i + 1];
+      int k = FL_FILT - 2;
+      while (k >= 0 && w[k] > in) {
+        w[k + 1] = w[k];
+        k--;
+      }
+      w[k + 1] = in;
+    }
+    med[j - FL_Z] = w[(FL_FILT + 1) / 2 - 1];
+  }
+}
+
 void dsaxfl_channflag(const float *spec, int nbeam, int nchan, float thr, const int *perm, int nperm,
                       int *mask) {
   const int nin = nchan - 2 * FL_Z;
@@ -80,10 +119,8 @@ void dsaxfl_channflag(const float *spec, int nbeam, int nchan, float thr, const 
   }
   for (int b = 0; b < nbeam; b++) {
     const float *row = spec + (size_t)b * nchan;
-    for (int j = FL_Z; j < nchan - FL_Z; j++) {
-      const int w0 = nchan - FL_Z - j >= FL_FILT ? j : nchan - FL_Z - FL_FILT;
-      cs[j - FL_Z] = row[j] - median_of(row + w0, FL_FILT, tmp);
-    }
+    window_medians(row, nchan, tmp);
+    for (int j = FL_Z; j < nchan - FL_Z; j++) cs[j - FL_Z] = row[j] - tmp[j - FL_Z];
     const float med = median_of(cs, nin, tmp);
     for (int j = 0; j < nin; j++) tmp[j] = fabsf(cs[j] - med);
     const float mad = select_k(tmp, nin, (nin + 1) / 2 - 1);

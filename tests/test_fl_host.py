@@ -50,6 +50,24 @@ def test_channel_masks_equal_the_oracle(seed):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("nc", [278, 300, 1024])
+def test_channflag_window_medians_with_ties_and_flat_runs(nc):
+    """channflag's 21-channel window medians (a sorted window sliding along
+    the band, clamped at the top edge) against the oracle's per-window sort:
+    heavy ties, a flat run longer than the window, a spike, the narrowest band
+    (nchan 278: 22 channels between the 128-channel edges)."""
+    rng = np.random.default_rng(nc)
+    nb = 4
+    spec = rng.normal(48, 2, (nb, nc)).astype(np.float32)
+    spec[0] = np.round(spec[0])                          # few distinct values
+    spec[1, 130:190] = 50.0                              # flat run
+    spec[2, nc // 2] += 25.0                             # spike
+    spec[3] = np.float32(47.5)                           # constant band: MAD 0
+    for thr in (2.0, 4.0):
+        a = fl.mask("channflag", spec, thr, perm=(10, 140))
+        assert np.array_equal(a, orc.fl_mask("channflag", spec, thr, perm=(10, 140)))
+
+
 def test_time_mask_gather_bpwr_and_median_calc_equal_the_oracle():
     rng = np.random.default_rng(4)
     nb, nt = 4, 16384
