@@ -11,8 +11,10 @@
  * double operation of the mask tests is kept as the reference writes it.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "dsaxfl.h"
 
@@ -58,6 +60,39 @@ static float select_k(float *a, int n, int k) {
 static float median_of(const float *a, int n, float *tmp) {
   memcpy(tmp, a, sizeof(float) * n);
   return select_k(tmp, n, (n + 1) / 2 - 1);
+}
+
+/* Beams [0, nbeam) in up to 8 host threads, each a contiguous range: every
+ * beam's arithmetic is its own, so the masks do not depend on the split. */
+typedef void (*fl_beam_fn)(const void *ctx, int b0, int b1);
+struct fl_range {
+  fl_beam_fn fn;
+  const void *ctx;
+  int b0, b1;
+};
+static void *fl_range_run(void *p) {
+  const struct fl_range *r = p;
+  r->fn(r->ctx, r->b0, r->b1);
+  return NULL;
+}
+static void fl_for_beams(int nbeam, fl_beam_fn fn, const void *ctx) {
+  const long ncpu = sysconf(_SC_NPROCESSORS_ONLN);
+  int nt = nbeam / 4 < 8 ? nbeam / 4 : 8;
+  if (ncpu > 0 && nt > ncpu) nt = (int)ncpu;
+  if (nt <= 1) {
+    fn(ctx, 0, nbeam);
+    return;
+  }
+  pthread_t th[8];
+  struct fl_range r[8];
+  int live[8] = {0};
+  for (int t = 0; t < nt; t++) r[t] = (struct fl_range){fn, ctx, (int)((long)nbeam * t / nt), (int)((long)nbeam * (t + 1) / nt)};
+  for (int t = 1; t < nt; t++) live[t] = pthread_create(&th[t], NULL, fl_range_run, &r[t]) == 0;
+  fn(ctx, r[0].b0, r[0].b1);
+  for (int t = 1; t < nt; t++) {
+    if (live[t]) pthread_join(th[t], NULL);
+    else fn(ctx, r[t].b0, r[t].b1);            /* no thread: this one does it */
+  }
 }
 
 float dsaxfl_medval(const float *a, int n) {
@@ -107,44 +142,58 @@ static void window_medians(const float *row, int nchan, float *med) {
   }
 }
 
-void dsaxfl_channflag(const float *spec, int nbeam, int nchan, float thr, const int *perm, int nperm,
-                      int *mask) {
-  const int nin = nchan - 2 * FL_Z;
-  if (nin < FL_FILT) return;
+struct fl_chan_ctx {
+  const float *spec;
+  int nchan;
+  float thr;
+  const int *perm;
+  int nperm;
+  int *mask;
+};
+static void channflag_beams(const void *vc, int b0, int b1) {
+  const struct fl_chan_ctx *c = vc;
+  const int nchan = c->nchan, nin = nchan - 2 * FL_Z;
   float *cs = malloc(sizeof(float) * nin), *tmp = malloc(sizeof(float) * nin);
   if (!cs || !tmp) {
     free(cs);
     free(tmp);
     return;
   }
-  for (int b = 0; b < nbeam; b++) {
-    const float *row = spec + (size_t)b * nchan;
+  for (int b = b0; b < b1; b++) {
+    const float *row = c->spec + (size_t)b * nchan;
     window_medians(row, nchan, tmp);
     for (int j = FL_Z; j < nchan - FL_Z; j++) cs[j - FL_Z] = row[j] - tmp[j - FL_Z];
     const float med = median_of(cs, nin, tmp);
     for (int j = 0; j < nin; j++) tmp[j] = fabsf(cs[j] - med);
     const float mad = select_k(tmp, nin, (nin + 1) / 2 - 1);
-    const float lim = thr * mad;
-    int *m = mask + (size_t)b * nchan;
+    const float lim = c->thr * mad;
+    int *m = c->mask + (size_t)b * nchan;
     for (int j = FL_Z; j < nchan - FL_Z; j++) {
       const float v = cs[j - FL_Z];
       if (v > lim || v < -lim) m[j] = 1;
     }
-    for (int k = 0; k < nperm; k++)
-      if (perm[k] >= FL_Z && perm[k] < nchan - FL_Z) m[perm[k]] = 1;
+    for (int k = 0; k < c->nperm; k++)
+      if (c->perm[k] >= FL_Z && c->perm[k] < nchan - FL_Z) m[c->perm[k]] = 1;
   }
   free(cs);
   free(tmp);
 }
 
-void dsaxfl_simple_channflag(const float *spec, int nbeam, int nchan, float thr, int *mask) {
-  const int nin = nchan - 2 * FL_Z;
-  if (nin < 1) return;
+void dsaxfl_channflag(const float *spec, int nbeam, int nchan, float thr, const int *perm, int nperm,
+                      int *mask) {
+  if (nchan - 2 * FL_Z < FL_FILT) return;
+  const struct fl_chan_ctx c = {spec, nchan, thr, perm, nperm, mask};
+  fl_for_beams(nbeam, channflag_beams, &c);
+}
+
+static void simple_channflag_beams(const void *vc, int b0, int b1) {
+  const struct fl_chan_ctx *c = vc;
+  const int nchan = c->nchan, nin = nchan - 2 * FL_Z;
   float *tmp = malloc(sizeof(float) * nin);
   if (!tmp) return;
-  const float mythr = (float)(thr / sqrt(1. * DSAXFL_FBIN));
-  for (int b = 0; b < nbeam; b++) {
-    const float *row = spec + (size_t)b * nchan;
+  const float mythr = (float)(c->thr / sqrt(1. * DSAXFL_FBIN));
+  for (int b = b0; b < b1; b++) {
+    const float *row = c->spec + (size_t)b * nchan;
     const float med = median_of(row + FL_Z, nin, tmp);
     for (int j = 0; j < nin; j++) tmp[j] = fabsf(row[FL_Z + j] - med);
     const float mad = select_k(tmp, nin, (nin + 1) / 2 - 1);
@@ -153,26 +202,38 @@ void dsaxfl_simple_channflag(const float *spec, int nbeam, int nchan, float thr,
       float vv = 0.f;
       for (int k = 0; k < DSAXFL_FBIN; k++) vv += row[j];   /* the reference adds spec[j] FBIN times */
       vv = (float)((double)vv / (1. * DSAXFL_FBIN) - (double)med);
-      if (vv > lim) mask[(size_t)b * nchan + j] = 1;
+      if (vv > lim) c->mask[(size_t)b * nchan + j] = 1;
     }
   }
   free(tmp);
 }
 
-void dsaxfl_simple_tsflag(const float *ts, int nbeam, int ntime, float thr, int *mask) {
-  const int nm = ntime / 16;
-  if (nm < 1) return;
+void dsaxfl_simple_channflag(const float *spec, int nbeam, int nchan, float thr, int *mask) {
+  if (nchan - 2 * FL_Z < 1) return;
+  const struct fl_chan_ctx c = {spec, nchan, thr, NULL, 0, mask};
+  fl_for_beams(nbeam, simple_channflag_beams, &c);
+}
+
+static void tsflag_beams(const void *vc, int b0, int b1) {
+  const struct fl_chan_ctx *c = vc;                  /* nchan = ntime here */
+  const int ntime = c->nchan, nm = ntime / 16;
   float *tmp = malloc(sizeof(float) * nm);
   if (!tmp) return;
-  for (int b = 0; b < nbeam; b++) {
-    const float *row = ts + (size_t)b * ntime;
+  for (int b = b0; b < b1; b++) {
+    const float *row = c->spec + (size_t)b * ntime;
     const float med = median_of(row, nm, tmp);
     for (int j = 0; j < nm; j++) tmp[j] = fabsf(row[j] - med);
-    const float lim = thr * select_k(tmp, nm, (nm + 1) / 2 - 1);
+    const float lim = c->thr * select_k(tmp, nm, (nm + 1) / 2 - 1);
     for (int j = 0; j < ntime; j++)
-      if (row[j] - med > lim) mask[(size_t)b * ntime + j] = 1;
+      if (row[j] - med > lim) c->mask[(size_t)b * ntime + j] = 1;
   }
   free(tmp);
+}
+
+void dsaxfl_simple_tsflag(const float *ts, int nbeam, int ntime, float thr, int *mask) {
+  if (ntime / 16 < 1) return;
+  const struct fl_chan_ctx c = {ts, ntime, thr, NULL, 0, mask};
+  fl_for_beams(nbeam, tsflag_beams, &c);
 }
 
 int dsaxfl_gather_mask(const int *mask, long n, int *idx) {

@@ -102,6 +102,9 @@ int dsaxfl_noise(uint8_t *d_data, int nbeam, int ntime, int nchan, const uint8_t
 int dsaxfl_fix_zspec(float *d_s0, float *d_v0, int nbc, int naver, void *stream);
 
 /* ---- CPU functions (gpu_flagger.cu:530-790) ----------------------------- */
+/* channflag, simple_channflag and simple_tsflag split their beams over up to
+ * 8 host threads (each beam's arithmetic unchanged, so the masks do not
+ * depend on the split); they keep no state and may be called concurrently. */
 
 /* medval: element (n+1)/2-1 of a sorted copy (the lower median). */
 float dsaxfl_medval(const float *a, int n);

@@ -68,6 +68,23 @@ def test_channflag_window_medians_with_ties_and_flat_runs(nc):
         assert np.array_equal(a, orc.fl_mask("channflag", spec, thr, perm=(10, 140)))
 
 
+def test_masks_over_many_beams_equal_the_oracle():
+    """64 beams (the reference's block), which the host functions split over
+    threads by beam ranges: every mask equals the oracle's single pass."""
+    rng = np.random.default_rng(64)
+    nb, nc, nt = 64, 1024, 4096
+    spec = rng.normal(48, 1.5, (nb, nc)).astype(np.float32)
+    spec[::7, 400:406] += 9
+    spec[5] = np.round(spec[5])
+    for kind in ("channflag", "simple_channflag"):
+        a = fl.mask(kind, spec, 4.0, perm=(10,))
+        assert np.array_equal(a, orc.fl_mask(kind, spec, 4.0, perm=(10,))) and a.sum() > nb
+    ts = rng.normal(40, 1, (nb, nt)).astype(np.float32)
+    ts[::5, 1000:1004] += 15
+    a = fl.mask("simple_tsflag", ts, 6.0)
+    assert np.array_equal(a, orc.fl_mask("simple_tsflag", ts, 6.0)) and a.sum() >= 4 * 13
+
+
 def test_time_mask_gather_bpwr_and_median_calc_equal_the_oracle():
     rng = np.random.default_rng(4)
     nb, nt = 4, 16384
