@@ -109,21 +109,24 @@ __global__ __launch_bounds__(256) void xc_turn_kernel(const uint8_t *__restrict_
   }
 }
 
-// The triangle's 32x32 tiles (J, I >= J) in J-major order, and the share of
-// them that wave part P of NS takes.
+// The triangle's 32x32 tiles (J <= I) in column-major order, (0,0), (0,1),
+// (1,1), (0,2), ..., and the share of them that wave part P of NS takes: in
+// this order the first shares need fewer antenna blocks (96 antennas, two
+// shares: blocks 0-1 with two W operands, and blocks 0-2 with one, where
+// row-major order gives 3 + 3 and 2 + 2).
 namespace {
 template <int NB> constexpr int ntile() { return NB * (NB + 1) / 2; }
 template <int NB> constexpr int tile_j(int t) {
   int k = 0;
-  for (int J = 0; J < NB; J++)
-    for (int I = J; I < NB; I++, k++)
+  for (int I = 0; I < NB; I++)
+    for (int J = 0; J <= I; J++, k++)
       if (k == t) return J;
   return -1;
 }
 template <int NB> constexpr int tile_i(int t) {
   int k = 0;
-  for (int J = 0; J < NB; J++)
-    for (int I = J; I < NB; I++, k++)
+  for (int I = 0; I < NB; I++)
+    for (int J = 0; J <= I; J++, k++)
       if (k == t) return I;
   return -1;
 }
@@ -165,11 +168,15 @@ __device__ __forceinline__ void xc_share(const XcArgs &a, int pq, int ub, int nu
     const int ant = 32 * b + n;
     off[b] = ant < a.nant ? (uint32_t)(ant * XU + 32 * h) : 0x80000000u;
   }
+  // (units past the slice read at offset in_bytes and up, which the buffer's
+  // range check answers with zeros: every pass of the loop below issues the
+  // same loads, so each unit waits only for its own)
   auto load = [&](int u, uint32_t (&r)[NB][8]) {
+    const uint32_t so = (uint32_t)__builtin_amdgcn_readfirstlane(
+        u < nunit ? (int)(((long)pq * ntot + ub + u) * a.nant * XU) : (int)a.in_bytes);
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       if (!reads<NB, NS, P>(b)) continue;
-      const uint32_t so = (uint32_t)__builtin_amdgcn_readfirstlane((int)(((long)pq * ntot + ub + u) * a.nant * XU));
       const u4v v0 = __builtin_amdgcn_raw_buffer_load_b128(rin, off[b], so, 0);
       const u4v v1 = __builtin_amdgcn_raw_buffer_load_b128(rin, off[b] + 16, so, 0);
       r[b][0] = v0.x; r[b][1] = v0.y; r[b][2] = v0.z; r[b][3] = v0.w;
@@ -215,19 +222,27 @@ __device__ __forceinline__ void xc_share(const XcArgs &a, int pq, int ub, int nu
   };
 
   uint32_t r[PD][NB][8];
+  // in unit order, as the loop issues them
 #pragma unroll
-  for (int k = 0; k < PD; k++)
-    if (k < nunit) load(k, r[k]);
-  for (int u0 = 0; u0 < nunit; u0 += PD) {
+  for (int k = 0; k < PD; k++) {
+    load(k, r[k]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // whole rounds of PD units, then the rest
+  int u0 = 0;
+  for (; u0 + PD <= nunit; u0 += PD) {
 #pragma unroll
     for (int k = 0; k < PD; k++) {
-      const int u = u0 + k;
-      if (u < nunit) {
-        compute(r[k]);
-        if (u + PD < nunit) load(u + PD, r[k]);
-      }
+      compute(r[k]);
+      load(u0 + k + PD, r[k]);
+      // (the loads stay here: sunk below the next units' work, they would be
+      // waited for with those units')
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
+#pragma unroll
+  for (int k = 0; k < PD - 1; k++)
+    if (u0 + k < nunit) compute(r[k]);
 
   // per-antenna sums over both lane halves: S = sum (ur + ui), E = sum (ur - ui)
   int S[NB], E[NB];
