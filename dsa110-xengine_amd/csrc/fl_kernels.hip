@@ -103,7 +103,23 @@ __global__ __launch_bounds__(1024) void fl_stats_kernel(FlStats a) {
   constexpr int PF = 16;
   static_assert(FL_TBIN % PF == 0, "bins end on a prefetch block");
   const size_t plane = (size_t)a.nt * nc;
-  const __amdgpu_buffer_rsrc_t rin = buffer_rsrc(a.data + (size_t)bm * plane, (uint32_t)plane);
+  // (the beam's base address through readfirstlane: the scaley path also uses
+  // bm in per-lane addresses, and a resource built from vector registers puts
+  // a waterfall loop around every load)
+  const uint64_t base = (uint64_t)(a.data + (size_t)bm * plane);
+  const uint64_t sbase = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)base) |
+                         (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(base >> 32)) << 32;
+  const __amdgpu_buffer_rsrc_t rin = buffer_rsrc((const void *)sbase, (uint32_t)plane);
+  // the scaled block through a buffer store over the same plane of wdata:
+  // non-live lanes' stores (offset past the plane) are dropped, so the rows
+  // need no branch
+  __amdgpu_buffer_rsrc_t rout = rin;
+  if (SCALE) {
+    const uint64_t wbase = (uint64_t)(a.wdata + (size_t)bm * plane);
+    const uint64_t swbase = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)wbase) |
+                            (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(wbase >> 32)) << 32;
+    rout = buffer_rsrc((const void *)swbase, (uint32_t)plane);
+  }
   const uint32_t cl = live ? (uint32_t)c0 : 0x80000000u;
   for (int h = 0; h < 2; h++) {
     const int seg = w + FL_SW * h;
@@ -117,13 +133,13 @@ __global__ __launch_bounds__(1024) void fl_stats_kernel(FlStats a) {
 #pragma unroll
       for (int d = 0; d < PF; d++) {
         uint32_t x = buf[d];
-        const uint32_t nx = (uint32_t)min(i0 + PF + d, per - 1);
-        buf[d] = __builtin_amdgcn_raw_buffer_load_b32(rin, cl + s0off + nx * (uint32_t)a.nc, 0, 0);
-        if (SCALE && live) {
+        if (SCALE) {
+          // (every lane: lane * 4 + 3 < FL_SLAB keeps a non-live lane's
+          // lookups in the table, and its statistics are never written)
           const uint8_t *t = lut + lane * 4;
           x = (uint32_t)t[(x & 0xFF) * FL_SLAB] | (uint32_t)t[((x >> 8) & 0xFF) * FL_SLAB + 1] << 8 |
               (uint32_t)t[((x >> 16) & 0xFF) * FL_SLAB + 2] << 16 | (uint32_t)t[(x >> 24) * FL_SLAB + 3] << 24;
-          *(uint32_t *)(a.wdata + (size_t)bm * plane + (size_t)(seg * per + i0 + d) * nc + c0) = x;
+          __builtin_amdgcn_raw_buffer_store_b32(x, rout, cl + s0off + (uint32_t)(i0 + d) * (uint32_t)a.nc, 0, 0);
         }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -132,6 +148,12 @@ __global__ __launch_bounds__(1024) void fl_stats_kernel(FlStats a) {
           Q[j] += v * v;
           M[j] = max(M[j], v);
         }
+        // the next load into buf[d] once x is dead, so that it can take x's
+        // register: a load issued while x is live needs another one, and the
+        // loop then rotates registers whose loads are still in flight
+        const uint32_t nx = (uint32_t)min(i0 + PF + d, per - 1);
+        buf[d] = __builtin_amdgcn_raw_buffer_load_b32(rin, cl + s0off + nx * (uint32_t)a.nc, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
       }
       if (((i0 + PF) & (FL_TBIN - 1)) == 0) {          // end of a 128-sample bin
 #pragma unroll
