@@ -63,9 +63,13 @@ def test_stats_match_oracle(orc, nb, nt, nc):
     assert np.array_equal(_get(d, x.shape, np.uint8), x)            # the block is only read
 
 
-def test_scaley_fused_with_stats_matches_oracle(orc):
-    nb, nt, nc = 3, 4096, 512
-    print("flagger scaley", flush=True)
+@pytest.mark.parametrize("nc", [512, 400, 272])
+def test_scaley_fused_with_stats_matches_oracle(orc, nc):
+    """scaley and the statistics of the scaled block; 400 and 272 channels end
+    in a part-filled 256-channel slab, whose lanes past the band must not
+    write (the whole block is compared, so a stray byte anywhere fails)."""
+    nb, nt = 3, 4096
+    print(f"flagger scaley {nb}x{nt}x{nc}", flush=True)
     rng = np.random.default_rng(5)
     x = _block(5, nb, nt, nc)
     spec0 = rng.uniform(20, 60, (nb, nc)).astype(np.float32)
