@@ -104,7 +104,10 @@ int dsaxfl_fix_zspec(float *d_s0, float *d_v0, int nbc, int naver, void *stream)
 /* ---- CPU functions (gpu_flagger.cu:530-790) ----------------------------- */
 /* channflag, simple_channflag and simple_tsflag split their beams over up to
  * 8 host threads (each beam's arithmetic unchanged, so the masks do not
- * depend on the split); they keep no state and may be called concurrently. */
+ * depend on the split); they keep no state and may be called concurrently.
+ * Like the reference's, they only set flagged entries to 1 and leave the
+ * others as they are, so several calls accumulate into one mask: zero it
+ * first, as the reference's loop does (gpu_flagger.cu:1339-1347,1388-1390). */
 
 /* medval: element (n+1)/2-1 of a sorted copy (the lower median). */
 float dsaxfl_medval(const float *a, int n);
