@@ -85,6 +85,40 @@ def test_masks_over_many_beams_equal_the_oracle():
     assert np.array_equal(a, orc.fl_mask("simple_tsflag", ts, 6.0)) and a.sum() >= 4 * 13
 
 
+def test_masks_set_flagged_entries_only_and_accumulate():
+    """The mask functions write 1 where they flag and leave every other entry
+    as it was (gpu_flagger.cu:597,642): over 64 beams (the threaded split) a
+    mask pre-filled with 7 keeps its 7s, and the reference loop's four calls
+    into one zeroed mask (gpu_flagger.cu:1339-1347) give the OR of the
+    oracle's four masks."""
+    rng = np.random.default_rng(77)
+    nb, nc, nt = 64, 1024, 4096
+    spec = rng.normal(48, 1.5, (nb, nc)).astype(np.float32)
+    spec[::5, 600:604] += 10
+    for kind, args in (("channflag", dict(perm=(10,))), ("simple_channflag", {})):
+        want = orc.fl_mask(kind, spec, 4.0, **args)
+        got = fl.mask(kind, spec, 4.0, out=np.full((nb, nc), 7, np.int32), **args)
+        assert np.array_equal(got, np.where(want == 1, 1, 7)) and want.sum() > nb
+    ts = rng.normal(40, 1, (nb, nt)).astype(np.float32)
+    ts[::3, 2000:2003] += 15
+    want = orc.fl_mask("simple_tsflag", ts, 6.0)
+    got = fl.mask("simple_tsflag", ts, 6.0, out=np.full((nb, nt), 7, np.int32))
+    assert np.array_equal(got, np.where(want == 1, 1, 7)) and want.sum() > 0
+    var = rng.normal(4, 0.3, (nb, nc)).astype(np.float32)
+    var[1::9, 300:302] += 3
+    mx = rng.normal(70, 2, (nb, nc)).astype(np.float32)
+    pp = rng.normal(5, 0.4, (nb, nc)).astype(np.float32)
+    pp[2::11, 800] += 4
+    m = np.zeros((nb, nc), np.int32)
+    m = fl.mask("channflag", spec, 5.0, perm=(10,), out=m)
+    m = fl.mask("channflag", var, 5.5, perm=(10,), out=m)
+    m = fl.mask("channflag", mx, 5.0, perm=(10,), out=m)
+    m = fl.mask("simple_channflag", pp, 5.0, out=m)
+    ors = (orc.fl_mask("channflag", spec, 5.0, perm=(10,)) | orc.fl_mask("channflag", var, 5.5, perm=(10,)) |
+           orc.fl_mask("channflag", mx, 5.0, perm=(10,)) | orc.fl_mask("simple_channflag", pp, 5.0))
+    assert np.array_equal(m, ors) and m.sum() > nb
+
+
 def test_time_mask_gather_bpwr_and_median_calc_equal_the_oracle():
     rng = np.random.default_rng(4)
     nb, nt = 4, 16384
