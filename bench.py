@@ -564,6 +564,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy, three times: the "
                          "known-size kernel that scales the HBM byte counters")
+    ap.add_argument("--copy-rate", action="store_true",
+                    help="time a plain 1 GiB device-to-device copy (the practical read + write "
+                         "rate that the block passes are compared with)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -578,6 +581,26 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n, "launches": 3}), flush=True)
+        return
+    if args.copy_rate:
+        import torch
+        n = 1 << 30
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        marks = [torch.cuda.Event(enable_timing=True) for _ in range(2)]   # torch's stream runs the copy
+        ms = []
+        for _ in range(20):
+            marks[0].record()
+            dst.copy_(src)
+            marks[1].record()
+            torch.cuda.synchronize()
+            ms.append(marks[0].elapsed_time(marks[1]))
+        med = float(np.median(ms))
+        print(json.dumps({"copy_bytes": n, "copies": 20, "copy_ms_median": med, "copy_ms_min": float(min(ms)),
+                          "read_plus_write_TB_s": 2 * n / (med / 1e3) / 1e12}), flush=True)
         return
 
     import dsaxbf
