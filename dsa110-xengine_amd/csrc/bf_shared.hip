@@ -32,7 +32,7 @@
 
 using namespace shared;
 
-template <int PREC, int TAVG>
+template <int PREC, int TAVG, bool GROUP>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void bf_shared_kernel(BfArgs a) {
   __shared__ __attribute__((aligned(16))) uint8_t stage[SSTAGE];
@@ -41,7 +41,7 @@ void bf_shared_kernel(BfArgs a) {
   const int P = blockIdx.x;
   const int L = (P & 7) * (gridDim.x >> 3) + (P >> 3);
   if (L >= a.ntiles) return;
-  shared_tile<PREC, TAVG>(a, L, stage, abuf);
+  shared_tile<PREC, TAVG, GROUP>(a, L, stage, abuf);
 }
 
 // Batched: the tiles of njobs blocks back to back; job j owns global tiles
@@ -59,13 +59,28 @@ void bf_shared_batch_kernel(const BfArgs *__restrict__ jobs, const int *__restri
   shared_tile<PREC, TAVG>(jobs[j], L - tile_start[j], stage, abuf);
 }
 
+template <int PREC, bool GROUP>
+static void launch_g(int tavg, dim3 grid, hipStream_t s, const BfArgs &a) {
+  switch (tavg) {
+    case 8:  hipLaunchKernelGGL((bf_shared_kernel<PREC, 8, GROUP>), grid, dim3(NT), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((bf_shared_kernel<PREC, 16, GROUP>), grid, dim3(NT), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((bf_shared_kernel<PREC, 32, GROUP>), grid, dim3(NT), 0, s, a); break;
+    default: hipLaunchKernelGGL((bf_shared_kernel<PREC, 64, GROUP>), grid, dim3(NT), 0, s, a); break;
+  }
+}
+
+// Group weights (one row per pol for a whole tile): the FP8 and FP6 hi+lo
+// splits at tavg 32 and 64 keep both rows in registers (245-252 VGPRs); FP16,
+// the three-part FP6 split and tavg 8 or 16 (more bin sums) would spill, so
+// they reload as for fine weights.
 template <int PREC>
 static void launch(int tavg, dim3 grid, hipStream_t s, const BfArgs &a) {
-  switch (tavg) {
-    case 8:  hipLaunchKernelGGL((bf_shared_kernel<PREC, 8>), grid, dim3(NT), 0, s, a); break;
-    case 16: hipLaunchKernelGGL((bf_shared_kernel<PREC, 16>), grid, dim3(NT), 0, s, a); break;
-    case 32: hipLaunchKernelGGL((bf_shared_kernel<PREC, 32>), grid, dim3(NT), 0, s, a); break;
-    default: hipLaunchKernelGGL((bf_shared_kernel<PREC, 64>), grid, dim3(NT), 0, s, a); break;
+  constexpr bool G = PREC == 1 || PREC == 2;
+  if (G && a.wmode == 0 && tavg >= 32) {
+    if (tavg == 32) hipLaunchKernelGGL((bf_shared_kernel<PREC, 32, G>), grid, dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL((bf_shared_kernel<PREC, 64, G>), grid, dim3(NT), 0, s, a);
+  } else {
+    launch_g<PREC, false>(tavg, grid, s, a);
   }
 }
 

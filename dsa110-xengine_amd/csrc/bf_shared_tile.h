@@ -243,8 +243,14 @@ template <int PREC> constexpr int abuf_bytes() { return PREC == 0 ? AB16 : 2 * A
 static_assert(2 * 4096 <= AB8, "FP6 operands of one (channel, pol) fit the FP8 half-buffer");
 
 // Tile L of the block described by a; stage and abuf are the workgroup's LDS.
-template <int PREC, int TAVG>
+// GROUP (weights per 8-channel group, FP8 / FP6 hi+lo): the tile's
+// (channel, pol) steps use two weight rows only, one per pol; both stay in
+// registers for the whole tile and the steps load no weights.  The same
+// MFMAs, detects and barriers as the rolling path, in the same order: the
+// results are identical.
+template <int PREC, int TAVG, bool GROUP = false>
 __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *stage, uint8_t *abuf) {
+  static_assert(!GROUP || PREC == 1 || PREC == 2, "FP16 and three-part FP6 keep the rolling weights");
   constexpr int UNIT = PREC == 0 ? AB16 / 2 : AB8 / 2;    // operand bytes per unit
   constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;
   const TileId t = tile_id(a, L);
@@ -264,6 +270,7 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
   // every older load) do not wait for them.  The second unit of each (channel,
   // pol) prefetches the next one's in place (the last one reloads itself).
   WFrag<PREC, BTW, 1> W;
+  WFrag<PREC, BTW, 1> W1;                 // pol 1's weights (GROUP)
 
   // (a) stage: the two (packet, antenna quad) tasks of each thread, every
   // load issued before the first store.  Buffer loads without branches: a
@@ -295,6 +302,10 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
       const long wr = weight_row(a, t.g, 0);
       __builtin_amdgcn_sched_barrier(0);
       W.load(a.wfrag, a.wscale, wr * a.nbt + btw, wr * a.nbeam_pad + btw * 32, lane, n);
+      if constexpr (GROUP) {
+        const long wr1 = weight_row(a, t.g, 1);
+        W1.load(a.wfrag, a.wscale, wr1 * a.nbt + btw, wr1 * a.nbeam_pad + btw * 32, lane, n);
+      }
       __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
@@ -317,7 +328,25 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
   // 8 KB halves; FP8: per (channel, pol), 8 KB halves), so each wave's unpack
   // of the next step interleaves with its MFMAs of this one and one barrier
   // per step publishes the next half and frees this one
-  if constexpr (PREC == 0) {
+  if constexpr (PREC == 0 && GROUP) {
+    unpack_unit_f16(stage, abuf, 0, tid);
+    __syncthreads();
+    auto step = [&](int cq, WFrag<PREC, BTW, 1> &Wc) {
+      Acc<BTW> A;
+      unpack_unit_f16(stage, abuf + UNIT, 2 * cq + 1, tid);
+      unit16<false>(A, Wc, abuf, lane, src, 0);
+      detect<TAVG>(A, Wc.sc, part[0]);
+      __syncthreads();
+      if (cq < 15) unpack_unit_f16(stage, abuf, 2 * cq + 2, tid);
+      unit16<false>(A, Wc, abuf + UNIT, lane, src, 0);
+      detect<TAVG>(A, Wc.sc, part[1]);
+      __syncthreads();
+    };
+    for (int cq = 0; cq < 16; cq += 2) {
+      step(cq, W);
+      step(cq + 1, W1);
+    }
+  } else if constexpr (PREC == 0) {
     unpack_unit_f16(stage, abuf, 0, tid);
     __syncthreads();
     for (int cq = 0; cq < 16; cq++) {
@@ -347,14 +376,28 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
     };
     unpack_cq(abuf, 0);
     __syncthreads();
-    for (int cq = 0; cq < 16; cq++) {
-      const uint32_t wn = weight_row(a, t.g, cq < 15 ? cq + 1 : cq);
-      if (cq < 15) unpack_cq(abuf + ((cq + 1) & 1) * AB8, cq + 1);
-      const uint8_t *ab = abuf + (cq & 1) * AB8;
-      unit8<PREC, false, NP>(pd, W, ab, lane, src, 0, part[1][1], part[0][0]);
-      unit8<PREC, true, NP>(pd, W, ab + UNIT, lane, src, wn * a.nbt + btw, part[0][1], part[1][0]);
-      W.load_scale_rb(src, wn * a.nbeam_pad + btw * 32);
-      __syncthreads();
+    if constexpr (GROUP) {
+      auto step = [&](int cq, WFrag<PREC, BTW, 1> &Wc) {
+        if (cq < 15) unpack_cq(abuf + ((cq + 1) & 1) * AB8, cq + 1);
+        const uint8_t *ab = abuf + (cq & 1) * AB8;
+        unit8<PREC, false, NP>(pd, Wc, ab, lane, src, 0, part[1][1], part[0][0]);
+        unit8<PREC, false, NP>(pd, Wc, ab + UNIT, lane, src, 0, part[0][1], part[1][0]);
+        __syncthreads();
+      };
+      for (int cq = 0; cq < 16; cq += 2) {
+        step(cq, W);
+        step(cq + 1, W1);
+      }
+    } else {
+      for (int cq = 0; cq < 16; cq++) {
+        const uint32_t wn = weight_row(a, t.g, cq < 15 ? cq + 1 : cq);
+        if (cq < 15) unpack_cq(abuf + ((cq + 1) & 1) * AB8, cq + 1);
+        const uint8_t *ab = abuf + (cq & 1) * AB8;
+        unit8<PREC, false, NP>(pd, W, ab, lane, src, 0, part[1][1], part[0][0]);
+        unit8<PREC, true, NP>(pd, W, ab + UNIT, lane, src, wn * a.nbt + btw, part[0][1], part[1][0]);
+        W.load_scale_rb(src, wn * a.nbeam_pad + btw * 32);
+        __syncthreads();
+      }
     }
     float sd[NP][4];
 #pragma unroll

@@ -245,6 +245,45 @@ def test_variants_bit_identical(bf, orc, prec, nant, nchan, npkt, nbeam, order, 
     _check(r, prec)
 
 
+@pytest.mark.parametrize("prec", [1, 2])
+@pytest.mark.parametrize("nant,nchan,npkt,nbeam,order,tavg,amode", [
+    (64, 16, 128, 256, 0, 32, 0),
+    (63, 24, 256, 512, 1, 64, 0),   # the reference's NANT, -r order, two beam sets
+    (48, 16, 128, 100, 0, 32, 1),
+])
+def test_group_weights_held_for_the_tile(bf, orc, prec, nant, nchan, npkt, nbeam, order, tavg, amode):
+    """Group weights (one row per pol per 8-channel group) at tavg 32 and 64:
+    the shared schedule keeps both rows in registers for the whole tile and
+    loads no weights in its (channel, pol) loop.  FP8: the same bits as every
+    classic schedule (which reload each step); FP8 and FP6: the oracle fed the
+    device's own weights."""
+    cfg = bf.config(nant=nant, nchan=nchan, npkt=npkt, nbeam=nbeam, tavg=tavg, precision=prec,
+                    weight_mode=0, angle_mode=amode, input_order=order)
+    antpos, cal, theta = bf.synth_geometry(nant, cfg.ncal, nbeam, 3 * nant + tavg, 1200.0, 0.1, 1.75)
+    freqs = bf.reference_freqs(1498.75, nchan)
+    block = bf.synth_block(cfg, 7 * nbeam + tavg)
+    geo = (antpos, cal, freqs, theta)
+    scale = np.full(nbeam, 2.0 ** -12, np.float32)
+    variants = [bf.VARIANT_SHARED_256] if prec == 2 else \
+        [v for v in range(bf.NVARIANT) if v != bf.VARIANT_RETIRED]
+    ref = [_run_variant(bf, cfg, block, geo, scale, v) for v in variants]
+    for k in range(1, len(variants)):
+        for a, b in zip(ref[0], ref[k]):
+            assert np.array_equal(a, b), bf.VARIANT_NAMES[variants[k]]
+    d = orc.dims(nant, nchan, npkt, nbeam, tavg, order)
+    W = orc.weights(d, 0, amode, antpos, cal, freqs, theta)
+    g = (0, min(2, nchan // 8))
+    shared = ref[variants.index(bf.VARIANT_SHARED_256)]
+    Wd = shared[3]
+    _check_weights(Wd, orc.quant_weights(d, 0, prec, W))
+    want = orc.power(d, 0, block, W, 0, *g)
+    r = dict(ref=want[:, :, :2], power=shared[1][:, :, :2], out=shared[0][:, :, :2],
+             ref_u8=orc.requant_block(want, scale)[:, :, :2],
+             ref_dev=orc.power(d, 0, block, Wd, 0, *g)[:, :, :2],
+             ref_enc=orc.power(d, 0, block, orc.quant_weights(d, 0, prec, W), 0, *g)[:, :, :2])
+    _check(r, prec)
+
+
 def test_autotune_picks_a_variant(bf):
     cfg = bf.config(nant=64, nchan=64, npkt=512, nbeam=256, tavg=32, weight_mode=1, angle_mode=1)
     antpos, cal, theta = bf.synth_geometry(64, 64, 256, 4)
