@@ -322,6 +322,27 @@ def stream_bench(dsaxbf, ev, nblocks=8, seed=77):
     }
 
 
+def group_weights_bench(dsaxbf, ev, freqs, blocks, steps=10, warmup=2, seed=2024):
+    """The same full blocks with group weights (one row per 8-channel group
+    and pol: the reference's calc_weights at freqs[8g+4], the CLI's default),
+    FP8-split and FP6-split on the shared schedule, whose tiles then keep both
+    pols' weight rows in registers and load no weights per (channel, pol)."""
+    cfg = dsaxbf.config(nant=NANT, nchan=NCHAN, npkt=NPKT, nbeam=NBEAM, tavg=TAVG,
+                        weight_mode=dsaxbf.WEIGHTS_GROUP, angle_mode=dsaxbf.ANGLE_SIN)
+    antpos, cal, theta = dsaxbf.synth_geometry(NANT, cfg.ncal, NBEAM, seed, 1200.0, 0.1, 1.75)
+    out = {"workload": "full DSA block as the headline, group weights (48 rows x 2 pols), tavg 32, "
+                       "shared schedule"}
+    for prec in ("fp8split", "fp6split"):
+        r = time_path(dsaxbf, ev, prec, cfg, antpos, cal, theta, freqs, blocks, steps, warmup,
+                      variant="shared256")
+        out[prec] = {"kernel_ms": r["kernel_ms"], "kernel_ms_min": r["kernel_ms_min"],
+                     "ms_per_step": r["ms_per_step"],
+                     "samples_per_s": SAMPLES_PER_BLOCK / (r["ms_per_step"] / 1e3),
+                     "mfma_frac_of_own_peak": FLOP_PER_BLOCK / (r["kernel_ms"] / 1e3) / 1e12
+                     / PEAK_TFLOPS[prec]}
+    return out
+
+
 def xengine_bench(dsaxbf, ev, reps=20, seed=91):
     """The correlator X-engine on a full block of the reference correlator's
     shape: 96 antennas x 384 channels x 4096 samples (2048 packets, 302 MB
@@ -559,6 +580,8 @@ def main():
                     help="skip the streamed scaled-array measurement (256 antennas, 1024 beams)")
     ap.add_argument("--no-xengine", action="store_true",
                     help="skip the correlator X-engine measurement (96 antennas, full block)")
+    ap.add_argument("--no-group", action="store_true",
+                    help="skip the group-weights measurement (FP8/FP6 splits, full block)")
     ap.add_argument("--no-flagger", action="store_true",
                     help="skip the RFI flagger measurement (64 beams x 16384 x 1024 block)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -652,6 +675,14 @@ def main():
             xeng = {"error": f"{type(exc).__name__}: {exc}"}
         print(f"# xengine: {xeng}", file=sys.stderr, flush=True)
 
+    group = None
+    if not args.no_group:
+        try:
+            group = group_weights_bench(dsaxbf, ev, freqs, blocks)
+        except Exception as exc:             # reported, never fatal to the headline line
+            group = {"error": f"{type(exc).__name__}: {exc}"}
+        print(f"# group weights: {group}", file=sys.stderr, flush=True)
+
     flag = None
     if not args.no_flagger:
         try:
@@ -710,6 +741,7 @@ def main():
         "tone_check": {k: errs[k] for k in ("tone_beam", "peak_beam_fp16") if k in errs},
         "scaled_array_stream": stream,
         "uneven_batch_graph": batch,
+        "group_weights": group,
         "correlator_xengine": xeng,
         "rfi_flagger": flag,
     }
