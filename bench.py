@@ -331,8 +331,8 @@ def group_weights_bench(dsaxbf, ev, freqs, blocks, steps=10, warmup=2, seed=2024
                         weight_mode=dsaxbf.WEIGHTS_GROUP, angle_mode=dsaxbf.ANGLE_SIN)
     antpos, cal, theta = dsaxbf.synth_geometry(NANT, cfg.ncal, NBEAM, seed, 1200.0, 0.1, 1.75)
     out = {"workload": "full DSA block as the headline, group weights (48 rows x 2 pols), tavg 32, "
-                       "shared schedule"}
-    for prec in ("fp8split", "fp6split"):
+                       "shared schedule (FP16 reloads its weights per step: held, they would spill)"}
+    for prec in ("fp16", "fp8split", "fp6split"):
         r = time_path(dsaxbf, ev, prec, cfg, antpos, cal, theta, freqs, blocks, steps, warmup,
                       variant="shared256")
         out[prec] = {"kernel_ms": r["kernel_ms"], "kernel_ms_min": r["kernel_ms_min"],
@@ -340,6 +340,8 @@ def group_weights_bench(dsaxbf, ev, freqs, blocks, steps=10, warmup=2, seed=2024
                      "samples_per_s": SAMPLES_PER_BLOCK / (r["ms_per_step"] / 1e3),
                      "mfma_frac_of_own_peak": FLOP_PER_BLOCK / (r["kernel_ms"] / 1e3) / 1e12
                      / PEAK_TFLOPS[prec]}
+    for prec in ("fp8split", "fp6split"):
+        out[f"{prec}_speedup_vs_fp16"] = out["fp16"]["ms_per_step"] / out[prec]["ms_per_step"]
     return out
 
 
