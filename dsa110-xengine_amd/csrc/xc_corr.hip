@@ -379,7 +379,9 @@ extern "C" int bfk_xcorr(int nant, int nchan, int npkt, int order, const uint8_t
   // two waves per plane above 32 antennas (measured on the 96-antenna block,
   // whole block: one, two, three and six waves per plane took 0.242, 0.238,
   // 0.249 and 0.259 ms; more waves per SIMD do not pay for the duplicated
-  // loads and conversions)
+  // loads and conversions.  With each unit waiting for its own loads: two
+  // waves 0.222 ms, three 0.252 ms, or 0.266 ms held to 168 registers for
+  // three waves per SIMD; XPD = 3 0.229 ms)
   const int ns = nant <= 32 ? 1 : 2;
   const long nwave = (long)ns * 2 * nchan * nslice;
   const dim3 grid((unsigned)(((nwave + 3) / 4 + 7) / 8 * 8)), block(256);
