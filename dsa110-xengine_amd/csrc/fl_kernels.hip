@@ -220,6 +220,13 @@ __device__ inline uint32_t fl_bytemask(uint32_t x) {
 // Replace the masked columns (mask[bm][ch] != 0) with the beam's byte and,
 // when ts is given, write ts[bm][t] = (sum over channels of the flagged row)
 // / nchan.  A wave per FL_RPW consecutive rows of one beam, 16 bytes per lane.
+// A lane stores its chunk when the chunk holds a masked column, or, when at
+// least a quarter of the wave's chunks do, every lane stores (unchanged chunks
+// are written back as read): on the 64 x 16384 x 1024 block, 5 % of columns
+// masked at random touch every line of a row, and whole rows then take
+// 0.436 ms against 0.459 ms as scattered chunks, while 0.5 % at random or a
+// 64-channel band run faster as chunks (0.357 and 0.343 ms, against 0.431
+// whole).
 __global__ __launch_bounds__(256) void fl_flag_ts_kernel(uint8_t *data, const uint8_t *mask,
                                                          const float *bpwr, float *ts, int nb, int nt,
                                                          int nc) {
@@ -240,7 +247,11 @@ __global__ __launch_bounds__(256) void fl_flag_ts_kernel(uint8_t *data, const ui
       m = *(const uint4 *)(mask + (size_t)bm * nc + 16 * q);
       m = make_uint4(fl_bytemask(m.x), fl_bytemask(m.y), fl_bytemask(m.z), fl_bytemask(m.w));
     }
-    const bool any = (m.x | m.y | m.z | m.w) != 0;
+    const bool own = (m.x | m.y | m.z | m.w) != 0;
+    // (both ballots in every active lane: under `own ||` they would run only
+    // in the lanes whose chunk is clean)
+    const bool whole = 4 * __popcll(__ballot(own)) >= __popcll(__ballot(1));
+    const bool any = own || whole;
 #pragma unroll
     for (int r = 0; r < FL_RPW; r++) {
       uint4 *pv = (uint4 *)(data + ((size_t)bm * nt + t0 + r) * nc + 16 * q);

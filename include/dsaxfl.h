@@ -62,7 +62,11 @@ int dsaxfl_scaley(uint8_t *d_data, int nbeam, int ntime, int nchan, const float 
                   const float *d_var0, float *d_spec, float *d_var, float *d_max, float *d_pp,
                   void *stream);
 
-/* make_ts (gpu_flagger.cu:413-452): ts[beam][time] = row sum / nchan. */
+/* make_ts (gpu_flagger.cu:413-452): ts[beam][time] = row sum / nchan, the
+ * exact integer sum over all nchan channels divided once.  The reference's 32
+ * threads take nchan/32 channels each (:419-429), so it sums them all only
+ * when nchan is a multiple of 32 (its NCHAN_P is 1024); other widths drop the
+ * last nchan % 32 channels there and not here. */
 int dsaxfl_make_ts(const uint8_t *d_data, int nbeam, int ntime, int nchan, float *d_ts, void *stream);
 
 /* flag (gpu_flagger.cu:458-474): every time of the n_mask columns

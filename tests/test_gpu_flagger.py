@@ -93,12 +93,17 @@ def test_scaley_fused_with_stats_matches_oracle(orc, nc):
     assert np.array_equal(_get(d2, x.shape, np.uint8), want)
 
 
-def test_flag_mask_with_make_ts_matches_oracle(orc):
-    nb, nt, nc = 4, 4096, 1024
-    print("flagger flag_mask + make_ts", flush=True)
+@pytest.mark.parametrize("nc", [1024, 400, 2048])
+def test_flag_mask_with_make_ts_matches_oracle(orc, nc):
+    nb, nt = 4, 4096
+    print(f"flagger flag_mask + make_ts nc={nc}", flush=True)
     rng = np.random.default_rng(6)
     x = _block(6, nb, nt, nc)
-    m = (rng.random((nb, nc)) < 0.05).astype(np.uint8)
+    # per beam: no mask, sparse (masked chunks stored alone), 5 % and 50 %
+    # (a quarter or more of a row's chunks: whole rows rewritten)
+    dens = np.array([0.0, 0.005, 0.05, 0.5])[:, None]
+    m = (rng.random((nb, nc)) < dens).astype(np.uint8)
+    m[1, 5] = 1
     m[2, 17] = 3                                         # any nonzero byte flags
     bp = np.array([1.0, 1.3, 0.7, 4.9], np.float32)      # 48 * 4.9 = 235.2
     d, d_m, d_bp = _dev(x), _dev(m), _dev(bp)
@@ -107,12 +112,25 @@ def test_flag_mask_with_make_ts_matches_oracle(orc):
     midx = np.flatnonzero(m.ravel()).astype(np.int32)
     want = orc.fl_flag(x, midx, bp)
     assert np.array_equal(_get(d, x.shape, np.uint8), want)
-    assert np.array_equal(_get(d_ts, (nb, nt), np.float32), orc.fl_stat("make_ts", want))
+    assert np.array_equal(_get(d_ts, (nb, nt), np.float32), _row_mean(want))
+    if nc % 32 == 0:
+        assert np.array_equal(_get(d_ts, (nb, nt), np.float32), orc.fl_stat("make_ts", want))
     # make_ts alone reads the block
     d2 = _dev(x)
     fl.make_ts(d2, nb, nt, nc, d_ts)
-    assert np.array_equal(_get(d_ts, (nb, nt), np.float32), orc.fl_stat("make_ts", x))
+    assert np.array_equal(_get(d_ts, (nb, nt), np.float32), _row_mean(x))
+    if nc % 32 == 0:
+        assert np.array_equal(_get(d_ts, (nb, nt), np.float32), orc.fl_stat("make_ts", x))
     assert np.array_equal(_get(d2, x.shape, np.uint8), x)
+
+
+def _row_mean(x):
+    """Every row's exact integer sum over all its channels / nchan, rounded
+    once to float.  The reference's make_ts (gpu_flagger.cu:419-429) gives
+    each of its 32 threads nchan/32 channels, so it equals this (and the
+    oracle) when nchan is a multiple of 32 and leaves the last nchan % 32
+    channels out otherwise; the library sums them all."""
+    return (x.sum(axis=2, dtype=np.int64) / x.shape[2]).astype(np.float32)
 
 
 def test_flag_and_flagts_by_index_match_oracle(orc):
