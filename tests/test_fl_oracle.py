@@ -50,6 +50,12 @@ def test_spectrum_and_make_ts_are_exact_integer_means():
     assert np.array_equal(s, (x.astype(np.int64).sum(axis=1) / NT).astype(np.float32))
     ts = orc.fl_stat("make_ts", x)
     assert np.array_equal(ts, (x.astype(np.int64).sum(axis=2) / NC).astype(np.float32))
+    # the reference's 32 threads of nchan/32 channels each (gpu_flagger.cu:419-429):
+    # at 400 channels the last 16 are left out of the sum, which is still
+    # divided by 400
+    y = _block(3, nc=400)
+    ts = orc.fl_stat("make_ts", y)
+    assert np.array_equal(ts, (y[:, :, :384].astype(np.int64).sum(axis=2) / 400).astype(np.float32))
 
 
 def test_varspec_exact_is_the_rounded_integer_variance():
