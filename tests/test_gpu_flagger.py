@@ -204,10 +204,11 @@ def _oracle_reference(orc, blocks):
     return orc.fl_median_calc(s[0]), orc.fl_median_calc(v[0])
 
 
-@pytest.mark.parametrize("pwr,mkrand,tsflag", [(1, 0, 1), (0, 0, 1), (1, 0, 0), (0, 1, 1)])
-def test_step_matches_oracle(orc, pwr, mkrand, tsflag):
-    nb, nt, nc = 4, 8192, 1024
-    print(f"flagger step pwr={pwr} mkrand={mkrand} tsflag={tsflag}", flush=True)
+@pytest.mark.parametrize("pwr,mkrand,tsflag,nc", [(1, 0, 1, 1024), (0, 0, 1, 1024), (1, 0, 0, 1024),
+                                                  (0, 1, 1, 1024), (1, 0, 1, 512)])
+def test_step_matches_oracle(orc, pwr, mkrand, tsflag, nc):
+    nb, nt = 4, 8192
+    print(f"flagger step pwr={pwr} mkrand={mkrand} tsflag={tsflag} nc={nc}", flush=True)
     cfg = fl.config(nb, nt, nc, pwr=pwr, mkrand=mkrand, tsflag=tsflag, tsthresh=6.0, seed=77)
     plan = fl.Plan(cfg)
     cal = [_block(20 + k, nb, nt, nc) for k in range(3)]
