@@ -325,13 +325,14 @@ def stream_bench(dsaxbf, ev, nblocks=8, seed=77):
 def group_weights_bench(dsaxbf, ev, freqs, blocks, steps=10, warmup=2, seed=2024):
     """The same full blocks with group weights (one row per 8-channel group
     and pol: the reference's calc_weights at freqs[8g+4], the CLI's default),
-    FP8-split and FP6-split on the shared schedule, whose tiles then keep both
-    pols' weight rows in registers and load no weights per (channel, pol)."""
+    on the shared schedule: the FP8 and FP6 splits keep both pols' weight rows
+    in registers and load no weights per (channel, pol); FP16 holds one row at
+    a time, its steps pol-outer, and loads the second once per tile."""
     cfg = dsaxbf.config(nant=NANT, nchan=NCHAN, npkt=NPKT, nbeam=NBEAM, tavg=TAVG,
                         weight_mode=dsaxbf.WEIGHTS_GROUP, angle_mode=dsaxbf.ANGLE_SIN)
     antpos, cal, theta = dsaxbf.synth_geometry(NANT, cfg.ncal, NBEAM, seed, 1200.0, 0.1, 1.75)
     out = {"workload": "full DSA block as the headline, group weights (48 rows x 2 pols), tavg 32, "
-                       "shared schedule (FP16 reloads its weights per step: held, they would spill)"}
+                       "shared schedule (FP8/FP6 splits hold both pols' rows; FP16 one at a time, pol-outer)"}
     for prec in ("fp16", "fp8split", "fp6split"):
         r = time_path(dsaxbf, ev, prec, cfg, antpos, cal, theta, freqs, blocks, steps, warmup,
                       variant="shared256")

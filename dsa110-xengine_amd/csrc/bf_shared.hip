@@ -32,7 +32,7 @@
 
 using namespace shared;
 
-template <int PREC, int TAVG, bool GROUP>
+template <int PREC, int TAVG, int GROUP>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void bf_shared_kernel(BfArgs a) {
   __shared__ __attribute__((aligned(16))) uint8_t stage[SSTAGE];
@@ -59,7 +59,7 @@ void bf_shared_batch_kernel(const BfArgs *__restrict__ jobs, const int *__restri
   shared_tile<PREC, TAVG>(jobs[j], L - tile_start[j], stage, abuf);
 }
 
-template <int PREC, bool GROUP>
+template <int PREC, int GROUP>
 static void launch_g(int tavg, dim3 grid, hipStream_t s, const BfArgs &a) {
   switch (tavg) {
     case 8:  hipLaunchKernelGGL((bf_shared_kernel<PREC, 8, GROUP>), grid, dim3(NT), 0, s, a); break;
@@ -70,17 +70,19 @@ static void launch_g(int tavg, dim3 grid, hipStream_t s, const BfArgs &a) {
 }
 
 // Group weights (one row per pol for a whole tile): the FP8 and FP6 hi+lo
-// splits at tavg 32 and 64 keep both rows in registers (245-252 VGPRs); FP16,
-// the three-part FP6 split and tavg 8 or 16 (more bin sums) would spill, so
-// they reload as for fine weights.
+// splits at tavg 32 and 64 keep both rows in registers (245-252 VGPRs);
+// FP16, the three-part FP6 split and tavg 8 or 16 (more bin sums) would
+// spill with both, so they hold one row at a time, pol-outer.
 template <int PREC>
 static void launch(int tavg, dim3 grid, hipStream_t s, const BfArgs &a) {
-  constexpr bool G = PREC == 1 || PREC == 2;
-  if (G && a.wmode == 0 && tavg >= 32) {
+  constexpr int G = PREC == 1 || PREC == 2 ? 1 : 2;
+  if (a.wmode != 0) {
+    launch_g<PREC, 0>(tavg, grid, s, a);
+  } else if (G == 1 && tavg >= 32) {
     if (tavg == 32) hipLaunchKernelGGL((bf_shared_kernel<PREC, 32, G>), grid, dim3(NT), 0, s, a);
     else hipLaunchKernelGGL((bf_shared_kernel<PREC, 64, G>), grid, dim3(NT), 0, s, a);
   } else {
-    launch_g<PREC, false>(tavg, grid, s, a);
+    launch_g<PREC, 2>(tavg, grid, s, a);
   }
 }
 

@@ -4,6 +4,8 @@
 // (bf_mixed.hip).
 #ifndef BF_SHARED_TILE_H
 #define BF_SHARED_TILE_H
+#include <type_traits>
+
 #include "bf_common.h"
 
 namespace shared {
@@ -243,14 +245,19 @@ template <int PREC> constexpr int abuf_bytes() { return PREC == 0 ? AB16 : 2 * A
 static_assert(2 * 4096 <= AB8, "FP6 operands of one (channel, pol) fit the FP8 half-buffer");
 
 // Tile L of the block described by a; stage and abuf are the workgroup's LDS.
-// GROUP (weights per 8-channel group, FP8 / FP6 hi+lo): the tile's
-// (channel, pol) steps use two weight rows only, one per pol; both stay in
-// registers for the whole tile and the steps load no weights.  The same
-// MFMAs, detects and barriers as the rolling path, in the same order: the
-// results are identical.
-template <int PREC, int TAVG, bool GROUP = false>
+// GROUP: weights per 8-channel group, so the tile's (channel, pol) steps use
+// two weight rows only, one per pol.
+//   0: the rolling weights of fine mode (each step loads the next step's row).
+//   1 (FP8 / FP6 hi+lo): both rows stay in registers for the whole tile and
+//      the steps load no weights.  The same MFMAs, detects and barriers as the
+//      rolling path, in the same order: the results are identical.
+//   2: the steps run pol-outer, (channel 0..7, pol 0) then (channel 0..7,
+//      pol 1), so one row is held at a time and the second is loaded in place
+//      once, during step 7's second unit, with the rolling path's registers.
+//      The powers are summed in another order than the rolling path's.
+template <int PREC, int TAVG, int GROUP = 0>
 __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *stage, uint8_t *abuf) {
-  static_assert(!GROUP || PREC == 1 || PREC == 2, "FP16 and three-part FP6 keep the rolling weights");
+  static_assert(GROUP != 1 || PREC == 1 || PREC == 2, "FP16 and three-part FP6 cannot hold both rows");
   constexpr int UNIT = PREC == 0 ? AB16 / 2 : AB8 / 2;    // operand bytes per unit
   constexpr int NP = TAVG <= 32 ? 32 / TAVG : 1;
   const TileId t = tile_id(a, L);
@@ -270,7 +277,7 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
   // every older load) do not wait for them.  The second unit of each (channel,
   // pol) prefetches the next one's in place (the last one reloads itself).
   WFrag<PREC, BTW, 1> W;
-  WFrag<PREC, BTW, 1> W1;                 // pol 1's weights (GROUP)
+  WFrag<PREC, BTW, 1> W1;                 // pol 1's weights (GROUP 1)
 
   // (a) stage: the two (packet, antenna quad) tasks of each thread, every
   // load issued before the first store.  Buffer loads without branches: a
@@ -302,7 +309,7 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
       const long wr = weight_row(a, t.g, 0);
       __builtin_amdgcn_sched_barrier(0);
       W.load(a.wfrag, a.wscale, wr * a.nbt + btw, wr * a.nbeam_pad + btw * 32, lane, n);
-      if constexpr (GROUP) {
+      if constexpr (GROUP == 1) {
         const long wr1 = weight_row(a, t.g, 1);
         W1.load(a.wfrag, a.wscale, wr1 * a.nbt + btw, wr1 * a.nbeam_pad + btw * 32, lane, n);
       }
@@ -328,24 +335,33 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
   // 8 KB halves; FP8: per (channel, pol), 8 KB halves), so each wave's unpack
   // of the next step interleaves with its MFMAs of this one and one barrier
   // per step publishes the next half and frees this one
-  if constexpr (PREC == 0 && GROUP) {
-    unpack_unit_f16(stage, abuf, 0, tid);
-    __syncthreads();
-    auto step = [&](int cq, WFrag<PREC, BTW, 1> &Wc) {
+  if constexpr (PREC == 0 && GROUP == 2) {
+    // step k takes (channel, pol) 2 (k % 8) + k / 8; steps 0-6 and 8-15 in
+    // loops that load no weights, step 7 between them loads pol 1's row (one
+    // loop with a branch on k == 7 would need copies of the fragments at the
+    // join and spill)
+    const uint32_t w1 = weight_row(a, t.g, 1);
+    auto cqof = [](int k) { return 2 * (k & 7) + (k >> 3); };
+    auto step = [&](int k, auto prefetch) {
+      const int cq = cqof(k);
       Acc<BTW> A;
       unpack_unit_f16(stage, abuf + UNIT, 2 * cq + 1, tid);
-      unit16<false>(A, Wc, abuf, lane, src, 0);
-      detect<TAVG>(A, Wc.sc, part[0]);
+      unit16<false>(A, W, abuf, lane, src, 0);
+      detect<TAVG>(A, W.sc, part[0]);
       __syncthreads();
-      if (cq < 15) unpack_unit_f16(stage, abuf, 2 * cq + 2, tid);
-      unit16<false>(A, Wc, abuf + UNIT, lane, src, 0);
-      detect<TAVG>(A, Wc.sc, part[1]);
+      if (k < 15) unpack_unit_f16(stage, abuf, 2 * cqof(k + 1), tid);
+      unit16<decltype(prefetch)::value>(A, W, abuf + UNIT, lane, src, w1 * a.nbt + btw);
+      detect<TAVG>(A, W.sc, part[1]);
+      if constexpr (decltype(prefetch)::value) W.load_scale_rb(src, w1 * a.nbeam_pad + btw * 32);
       __syncthreads();
     };
-    for (int cq = 0; cq < 16; cq += 2) {
-      step(cq, W);
-      step(cq + 1, W1);
-    }
+    unpack_unit_f16(stage, abuf, 0, tid);
+    __syncthreads();
+#pragma unroll 1
+    for (int k = 0; k < 7; k++) step(k, std::false_type{});
+    step(7, std::true_type{});
+#pragma unroll 1
+    for (int k = 8; k < 16; k++) step(k, std::false_type{});
   } else if constexpr (PREC == 0) {
     unpack_unit_f16(stage, abuf, 0, tid);
     __syncthreads();
@@ -376,7 +392,7 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
     };
     unpack_cq(abuf, 0);
     __syncthreads();
-    if constexpr (GROUP) {
+    if constexpr (GROUP == 1) {
       auto step = [&](int cq, WFrag<PREC, BTW, 1> &Wc) {
         if (cq < 15) unpack_cq(abuf + ((cq + 1) & 1) * AB8, cq + 1);
         const uint8_t *ab = abuf + (cq & 1) * AB8;
@@ -388,6 +404,24 @@ __device__ __forceinline__ void shared_tile(const BfArgs &a, int L, uint8_t *sta
         step(cq, W);
         step(cq + 1, W1);
       }
+    } else if constexpr (GROUP == 2) {
+      // step k takes (channel, pol) 2 (k % 8) + k / 8, laid out as in the
+      // FP16 branch
+      const uint32_t w1 = weight_row(a, t.g, 1);
+      auto step = [&](int k, auto prefetch) {
+        if (k < 15) unpack_cq(abuf + ((k + 1) & 1) * AB8, 2 * ((k + 1) & 7) + ((k + 1) >> 3));
+        const uint8_t *ab = abuf + (k & 1) * AB8;
+        unit8<PREC, false, NP>(pd, W, ab, lane, src, 0, part[1][1], part[0][0]);
+        unit8<PREC, decltype(prefetch)::value, NP>(pd, W, ab + UNIT, lane, src, w1 * a.nbt + btw, part[0][1],
+                                                   part[1][0]);
+        if constexpr (decltype(prefetch)::value) W.load_scale_rb(src, w1 * a.nbeam_pad + btw * 32);
+        __syncthreads();
+      };
+#pragma unroll 1
+      for (int k = 0; k < 7; k++) step(k, std::false_type{});
+      step(7, std::true_type{});
+#pragma unroll 1
+      for (int k = 8; k < 16; k++) step(k, std::false_type{});
     } else {
       for (int cq = 0; cq < 16; cq++) {
         const uint32_t wn = weight_row(a, t.g, cq < 15 ? cq + 1 : cq);

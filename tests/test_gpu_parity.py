@@ -284,6 +284,44 @@ def test_group_weights_held_for_the_tile(bf, orc, prec, nant, nchan, npkt, nbeam
     _check(r, prec)
 
 
+@pytest.mark.parametrize("prec,tavg", [(0, 16), (0, 32), (3, 16), (3, 32), (1, 16), (2, 8)])
+def test_group_weights_pol_outer(bf, orc, prec, tavg):
+    """Group weights where both rows do not fit in registers (FP16, the
+    three-part FP6 split, tavg 8 and 16): the shared schedule runs its
+    (channel, pol) steps pol-outer and holds one row at a time.  Its powers are
+    the classic schedule's (which reloads every step) summed over (channel,
+    pol) in another order, so they agree to FP32 rounding (16 positive terms:
+    under 2e-6 relative; FP16 and FP8, as the FP6 splits run on shared tiles
+    only), and they match the oracle fed the device's weights."""
+    nant, nchan, npkt, nbeam = 64, 24, 256, 256
+    cfg = bf.config(nant=nant, nchan=nchan, npkt=npkt, nbeam=nbeam, tavg=tavg, precision=prec,
+                    weight_mode=0, angle_mode=0)
+    antpos, cal, theta = bf.synth_geometry(nant, cfg.ncal, nbeam, 5 + tavg + prec, 1200.0, 0.1, 1.75)
+    freqs = bf.reference_freqs(1498.75, nchan)
+    block = bf.synth_block(cfg, 11 * tavg + prec)
+    geo = (antpos, cal, freqs, theta)
+    scale = np.full(nbeam, 2.0 ** -12, np.float32)
+    sh = _run_variant(bf, cfg, block, geo, scale, bf.VARIANT_SHARED_256)
+    if prec < 2:
+        cl = _run_variant(bf, cfg, block, geo, scale, bf.VARIANT_CLASSIC_256)
+        for a, b in ((sh[1], cl[1]), (sh[2], cl[2])):      # power, beam power
+            a, b = a.astype(np.float64), b.astype(np.float64)
+            assert (np.abs(a - b) <= 4e-6 * np.abs(b) + 1e-30).all()
+        du = _u8_dist(sh[0], cl[0])
+        assert du.max() <= 1 and (du > 0).mean() <= 1e-3
+    d = orc.dims(nant, nchan, npkt, nbeam, tavg, 0)
+    W = orc.weights(d, 0, 0, antpos, cal, freqs, theta)
+    g = (0, 2)
+    Wd = sh[3]
+    _check_weights(Wd, orc.quant_weights(d, 0, prec, W))
+    want = orc.power(d, 0, block, W, 0, *g)
+    r = dict(ref=want[:, :, :2], power=sh[1][:, :, :2], out=sh[0][:, :, :2],
+             ref_u8=orc.requant_block(want, scale)[:, :, :2],
+             ref_dev=orc.power(d, 0, block, Wd, 0, *g)[:, :, :2],
+             ref_enc=orc.power(d, 0, block, orc.quant_weights(d, 0, prec, W), 0, *g)[:, :, :2])
+    _check(r, prec)
+
+
 def test_autotune_picks_a_variant(bf):
     cfg = bf.config(nant=64, nchan=64, npkt=512, nbeam=256, tavg=32, weight_mode=1, angle_mode=1)
     antpos, cal, theta = bf.synth_geometry(64, 64, 256, 4)
